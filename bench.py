@@ -1,0 +1,165 @@
+"""Throughput benchmark: PCISPH particle-updates/s on one MI355X.
+
+Workload: the 1M-particle PCISPH throughput scene (0.005 m lattice, h =
+0.01 m, Morton re-sort every step, 2x1x1 m tank sampled as one boundary
+layer; fluids_amd.scenes.c3_pcisph_throughput). A step is one full
+simulate() including every PCISPH correction; value = N * K / wall time of
+K timed steps after W warm-up steps, with all state resident in HBM.
+
+Roofline: algorithmic bytes per particle-update follow the unique-bytes model
+of the reference dataflow (each array a reference kernel reads or writes
+counted once per launch, neighbour gathers as cache hits, list indices int32
+at the measured mean neighbour counts K and Kb). The dominant kernel's
+achieved bandwidth is its algorithmic bytes per launch over its mean launch
+time measured with HIP events on the solver's stream inside the timed region.
+
+cpu_baseline: the CPU restatement of the reference kernels (oracle/, fp32,
+same kernel sequence, OpenMP) on the same scene, timed over a few steps.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def kernel_bytes(name, K, Kb, n_it):
+    """Algorithmic bytes per particle of one launch of each kernel (the
+    reference kernels each one replaces)."""
+    table = {
+        "hash": 24.0,
+        "radix_scatter": 16.0 / 4,  # the reference's sort counts 16 B/p over all passes
+        "radix_hist": 0.0,
+        "radix_scan": 0.0,
+        "gather_ranges": 72.0 + 21.0,
+        "lists_density": 68.0 + 8 * K + 8 * Kb,
+        "normals": 40.0 + 4 * K,
+        "forces_predict": 72.0 + 80.0 + 16.0 + 4 * K,
+        "update_pressure": 40.0 + 4 * K + 4 * Kb + 4.0 / max(n_it, 1),
+        "pressure_force": 48.0 + 4 * K + 4 * Kb + (80.0 * (n_it - 1) + 96.0) / max(n_it, 1),
+    }
+    return table.get(name)
+
+
+def step_bytes(K, Kb, n_it, N, B, B_moving):
+    """B_pci per particle-update, reference dataflow."""
+    b = 429 + 168 * n_it + (16 + 8 * n_it) * K + (8 + 8 * n_it) * Kb
+    return b + (108.0 * B + 132.0 * B_moving) / N
+
+
+def run_cpu_baseline(sc, steps, threads):
+    import oracle
+    oracle.set_threads(threads)
+    o = oracle.Oracle(sc.settings)
+    o.set_rect_limits(*sc.container)
+    for v in sc.volumes:
+        o.add_volume(v)
+    for b in sc.bodies:
+        o.add_boundary(b.samples, b.can_move, b.mass, b.R, b.com)
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    return sc.n_fluid * steps / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-events", action="store_true", help="time without per-kernel events")
+    ap.add_argument("--json-out", default=None)
+    a = ap.parse_args()
+
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}); the solver has no multi-GPU path. "
+              "See DESIGN.md for how a scene would shard.", file=sys.stderr)
+        sys.exit(2)
+
+    from fluids_amd import scenes
+    from fluids_amd._lib import lib
+    lib()
+    t_init = time.perf_counter()
+    sc = scenes.c3_pcisph_throughput(a.n)
+    sim = scenes.build(sc)
+    sim.synchronize()
+    t_init = time.perf_counter() - t_init
+    print(f"# scene {sc.name}: N={sc.n_fluid} B={sc.n_boundary} init {t_init:.1f}s", file=sys.stderr, flush=True)
+
+    sim.step(a.warmup)
+    sim.synchronize()
+    st = sim.step_stats()
+    print(f"# warmup done: {st}", file=sys.stderr, flush=True)
+
+    sim.reset_kernel_times()
+    sim.set_profiling(not a.no_events)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.step(a.steps)
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+    sim.set_profiling(False)
+    kt = sim.kernel_times()
+    st = sim.step_stats()
+
+    N, B = sc.n_fluid, sc.n_boundary
+    n_it = st["iterations"]
+    K = st["fluid_neighbours"] / N
+    Kb = st["boundary_neighbours"] / N
+    pu_s = N * a.steps / wall
+    bpp = step_bytes(K, Kb, n_it, N, B, 0)
+
+    # dominant kernel by total time
+    per = {}
+    for name, (launches, ms) in kt.items():
+        kb = kernel_bytes(name, K, Kb, n_it)
+        per[name] = dict(launches=launches, total_ms=ms, avg_us=1e3 * ms / max(launches, 1),
+                         alg_bytes_per_launch=None if kb is None else kb * N)
+    dom = max(per, key=lambda k: per[k]["total_ms"]) if per else None
+    roof = None
+    if dom is not None and per[dom]["alg_bytes_per_launch"]:
+        avg_s = per[dom]["avg_us"] * 1e-6
+        achieved = per[dom]["alg_bytes_per_launch"] / avg_s
+        roof = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": None,
+                "alg_bytes_per_launch": per[dom]["alg_bytes_per_launch"], "avg_launch_us": per[dom]["avg_us"]}
+
+    cpu = None
+    if not a.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        v, dt = run_cpu_baseline(sc, a.cpu_steps, threads)
+        cpu = {"value": v, "unit": "particle-updates/s", "cores": threads, "kind": "port",
+               "sample": f"{a.cpu_steps} PCISPH steps of the same {N}-particle scene after its relaxation step "
+                         f"({dt:.1f} s), fp32 oracle, OpenMP"}
+
+    rec = {
+        "metric": "particle-updates/s (PCISPH 1M fp32), %HBM roofline; batched & adjoint vs plain",
+        "value": pu_s, "unit": "particle-updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"pcisph_throughput_{N // 1000}k", "fluid_particles": N, "boundary_particles": B,
+                   "spacing_m": 0.005, "support_radius_m": 0.01, "corrections": n_it,
+                   "mean_fluid_neighbours": round(K, 3), "mean_boundary_neighbours": round(Kb, 3)},
+        "roofline": roof,
+        "step_roofline": {"alg_bytes_per_pu": bpp, "achieved_GBps": bpp * pu_s / 1e9, "frac": bpp * pu_s / HBM_PEAK},
+        "cpu_baseline": cpu,
+        "kernels": per,
+        "events_in_timed_region": not a.no_events,
+    }
+    line = json.dumps(rec)
+    print(line, flush=True)
+    if a.json_out:
+        with open(a.json_out, "w") as f:
+            f.write(line + "\n")
+
+
+if __name__ == "__main__":
+    main()

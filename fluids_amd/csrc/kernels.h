@@ -1,0 +1,147 @@
+// Host-side launchers of the gfx950 SPH kernels and their parameter blocks.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace sph {
+
+struct GridDev;
+struct DevState;
+
+// Per-kernel HIP-event timing on the launching stream. Events are recorded
+// around each launch while enabled and resolved lazily.
+class Profiler {
+public:
+    bool enabled = false;
+    void begin(const char* name, hipStream_t s);
+    void end(hipStream_t s);
+    void resolve();  // waits for pending events and accumulates their times
+    void reset();
+    struct Entry {
+        std::string name;
+        long long launches = 0;
+        double total_ms = 0.0;
+    };
+    std::vector<Entry> entries;
+    ~Profiler();
+
+private:
+    struct Pending {
+        int entry;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending_;
+    std::vector<hipEvent_t> pool_;
+    int current_ = -1;
+    hipEvent_t cur_a_ = nullptr;
+    hipEvent_t get_event();
+};
+
+#define PROF_BEGIN(p, name, s) \
+    do {                        \
+        if ((p) && (p)->enabled) (p)->begin(name, s); \
+    } while (0)
+#define PROF_END(p, s) \
+    do {               \
+        if ((p) && (p)->enabled) (p)->end(s); \
+    } while (0)
+
+// Density pass parameters: PCISPH uses H2 (SUPPORT_RADIUS literal squared),
+// M = PARTICLE_MASS literal, w_eval = MullerConstants::default_eval;
+// WCSPH uses h (full support radius argument), M = particle mass argument,
+// w_eval = poly6_eval, rho0 and k_gas for the Tait pressure.
+struct DensityParams {
+    float h, H2, M, w_eval, rho0, k_gas;
+    int fix_density;
+};
+
+// PCISPH constants as the reference passes them (literals for -D values,
+// full fp32 for kernel arguments).
+struct PciParams {
+    float H, H2, M, rho0_d;          // SUPPORT_RADIUS, its square, PARTICLE_MASS, REST_DENSITY literals
+    float w_eval, w_grad, visc_lapl, press_grad, st_main, st_term;
+    float k_visc, sigma, delta;
+    float dt, max_vel, gx, gy, gz;
+    float lo_x, hi_x, lo_y, lo_z, hi_z;  // container clamp: (-c)/2 and c/2; y is clamped to [lo_y, 10]
+    float rho0_host, eta;                // exit test operands (host floats)
+    int min_iter, max_iter;
+};
+
+struct WcParams {
+    float h, M, spiky_grad, visc_lapl, k_visc, sigma, rho0, st_main, st_term;
+    float dt, max_vel, gx, gy, gz;
+    float lo_x, hi_x, lo_y, lo_z, hi_z;
+};
+
+struct BodyDev {
+    int origin, count, partial_offset, nblocks;
+    float com[3];
+    float phi_mass;  // unused slot kept for alignment
+};
+
+// grid.hip
+int sort_passes(int key_bits);
+size_t sort_hist_words(int n);
+void launch_hash(int n, const float4* pos, const GridDev& g, uint32_t* keys, DevState* st, hipStream_t s,
+                 Profiler* prof);
+void launch_radix_sort(uint32_t* keys_a, uint32_t* keys_b, int* vals_a, int* vals_b, uint32_t* hist, int n,
+                       int key_bits, hipStream_t s, Profiler* prof);
+void launch_gather_fluid(int n, const int* perm, const uint32_t* keys_s, const float4* pos, const float4* vel,
+                         const float4* vh, float4* pos_s, float4* vel_s, float4* vh_s, int4* R, const GridDev& g,
+                         int epoch, hipStream_t s, Profiler* prof);
+void launch_gather_boundary(int n, const int* perm, const uint32_t* keys_s, const float4* pos, const float* phi,
+                            float4* pos_s, float* phi_s, int4* R, const GridDev& g, int epoch, hipStream_t s,
+                            Profiler* prof);
+void launch_lists_density(bool wcsph, int n, const float4* pos_s, const int4* fR, int fepoch, int nb,
+                          const float4* bpos_s, const float* bphi_s, const int4* bR, int bepoch, const GridDev& g,
+                          const DensityParams& P, int* nl, int* nc, int* bnl, int* bnc, float* density,
+                          float* pressure, DevState* st, hipStream_t s, Profiler* prof);
+void launch_normals(int n, const float4* pos_s, const float* density, const int* nl, const int* nc, float H2,
+                    float w_grad, float M, float H, float4* normal, hipStream_t s, Profiler* prof);
+void launch_step_begin(DevState* st, hipStream_t s);
+void launch_fill_u32(void* p, size_t n, uint32_t v, hipStream_t s);
+void launch_stats(int n, const int* nc, const int* bnc, const float4* pos, unsigned long long* acc, hipStream_t s);
+
+// pcisph.hip
+void launch_forces_predict(int n, const float4* pos_s, const float4* vel_s, const float* density,
+                           const float4* normal, const int* nl, const int* nc, const PciParams& P, float4* f_other,
+                           float4* pos_pred, hipStream_t s, Profiler* prof);
+void launch_update_pressure(int n, int iter, const float4* pos_pred, const int* nl, const int* nc, const int* bnl,
+                            const int* bnc, const float4* bpos_s, const float* bphi_s, int has_b,
+                            const PciParams& P, float* err, float* pressure, DevState* st, hipStream_t s,
+                            Profiler* prof);
+void launch_pressure_force(int n, int iter, const float4* pos_s, const float4* vel_s, const float* density,
+                           const float* pressure, const float4* f_other, const int* nl, const int* nc,
+                           const int* bnl, const int* bnc, const float4* bpos_s, const float* bphi_s, int has_b,
+                           const PciParams& P, float4* f_press, float4* pos_pred, float4* pos_out, float4* vel_out,
+                           DevState* st, hipStream_t s, Profiler* prof);
+void launch_pci_integrate_only(int n, const float4* pos_s, const float4* vel_s, const float4* f_other,
+                               const PciParams& P, float4* pos_out, float4* vel_out, hipStream_t s,
+                               Profiler* prof);
+
+// wcsph.hip
+void launch_accel_integrate(int n, const float4* pos_s, const float4* vel_s, const float4* vh_s,
+                            const float* density, const float* pressure, const float4* normal, const int* nl,
+                            const int* nc, const int* bnl, const int* bnc, const float4* bpos_s,
+                            const float* bphi_s, const WcParams& P, float4* accel, float4* pos_out,
+                            float4* vel_out, float4* vh_out, hipStream_t s, Profiler* prof);
+
+// boundary.hip
+void launch_transform(int n, const float4* raw, float4* out, const float R[9], const float com[3], hipStream_t s,
+                      Profiler* prof);
+void launch_phi(int n, const float4* p, float H_d, float w_eval, float rho0, float phi_coeff, float* phi,
+                hipStream_t s, Profiler* prof);
+int fluid_force_blocks(int count);
+void launch_fluid_force(int count, int origin, const float4* b_unsorted, const float* b_phi, int n,
+                        const float4* pos_s, const float* density, const float* pressure, const int4* fR, int fepoch,
+                        const GridDev& g, float H, float spiky_grad, float mass, const float com[3],
+                        float4* b_force, float4* b_torque, float* partial, DevState* st, hipStream_t s,
+                        Profiler* prof);
+void launch_wrench_reduce(int nbodies, const int* offsets, const int* nblocks, const float* partial, float* wrench,
+                          hipStream_t s, Profiler* prof);
+
+}  // namespace sph

@@ -1,0 +1,264 @@
+// PCISPH kernels on gfx950, restating kernels/pcisph.cl and the correction
+// loop of PCISPHSimulation::_simulate_pcisph_step (pcisphsimluation.cpp:72-165).
+//
+// Launch structure of one step (after neighbour search and density):
+//   forces_predict       compute_initial_forces (pcisph.cl:134-222) + the
+//                        first predict_pos (pcisph.cl:302-344, F_p = 0)
+//   per correction i:
+//     update_pressure    pcisph.cl:349-418, plus the max of the density error
+//                        (the reference's clmax, pcisphsimluation.cpp:471-476)
+//                        reduced over each wave and merged with one atomic
+//     pressure_force     pcisph.cl:420-504, then either the next predict_pos
+//                        or, when this correction is the last one, the final
+//                        predict_vel_n_pos (pcisph.cl:240-284)
+// The exit test (pcisphsimluation.cpp:142-147) is evaluated on the device by
+// every thread of pressure_force from the reduced maximum, so the loop runs
+// without a host round trip; corrections launched past the exit return at once.
+#include "kernels.h"
+#include "sph_device.h"
+
+namespace sph {
+
+// acc = g + (pressure_force + other_force) / PARTICLE_MASS (pcisph.cl:260,321)
+__device__ __forceinline__ V3 pci_accel(V3 fp, V3 fo, const PciParams& P) {
+    return {P.gx + (fp.x + fo.x) / P.M, P.gy + (fp.y + fo.y) / P.M, P.gz + (fp.z + fo.z) / P.M};
+}
+
+// predict_pos body (pcisph.cl:319-343)
+__device__ __forceinline__ V3 pci_predict(V3 x, V3 v, V3 acc, const PciParams& P) {
+    v = v + clamp3(acc * P.dt, -P.max_vel, P.max_vel);
+    x = x + v * P.dt;
+    float vn = length(v);
+    V3 cp = {cl_clamp(x.x, P.lo_x, P.hi_x), cl_clamp(x.y, P.lo_y, 10.0f), cl_clamp(x.z, P.lo_z, P.hi_z)};
+    float d = length(cp - x);
+    if (d > 0.0f && vn > 0.0f) x = cp;
+    return x;
+}
+
+// predict_vel_n_pos body (pcisph.cl:258-283)
+__device__ __forceinline__ void pci_integrate(V3& x, V3& v, V3 acc, const PciParams& P) {
+    v = v + clamp3(acc * P.dt, -P.max_vel, P.max_vel);
+    x = x + v * P.dt;
+    float vn = length(v);
+    V3 cp = {cl_clamp(x.x, P.lo_x, P.hi_x), cl_clamp(x.y, P.lo_y, 10.0f), cl_clamp(x.z, P.lo_z, P.hi_z)};
+    V3 dv = cp - x;
+    float d = length(dv);
+    if (d > 0.0f && vn > 0.0f) {
+        V3 nrm = normalize(V3{cl_sign(dv.x), cl_sign(dv.y), cl_sign(dv.z)});
+        float rest = d / (P.dt * vn);
+        float k = (1.0f + rest) * dot(v, nrm);
+        v = v - k * nrm;
+        x = cp;
+    }
+}
+
+// surface tension spline st_kernel (kernels/surface_tension.h:23-36)
+__device__ __forceinline__ float st_kernel(float r, float h, float c) {
+    if (r <= 0.5f * h) {
+        float t1 = h - r;
+        float t2 = cube(t1) * cube(r);
+        return 2.0f * t2 - c;
+    } else if (r <= h) {
+        float t1 = h - r;
+        return cube(t1) * cube(r);
+    }
+    return 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_forces_predict(int n, const float4* __restrict__ pos_s,
+                                                        const float4* __restrict__ vel_s,
+                                                        const float* __restrict__ density,
+                                                        const float4* __restrict__ normal,
+                                                        const int* __restrict__ nl, const int* __restrict__ nc,
+                                                        PciParams P, float4* __restrict__ f_other,
+                                                        float4* __restrict__ pos_pred) {
+    int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    V3 pi = v3(pos_s[i]), vi = v3(vel_s[i]);
+    float di = density[i];
+    V3 ni = v3(normal[i]);
+    V3 fv = {0.0f, 0.0f, 0.0f}, fcurv = {0.0f, 0.0f, 0.0f}, fcoh = {0.0f, 0.0f, 0.0f};
+    int len = nc[i];
+    for (int k = 0; k < len; ++k) {
+        int j = nl[(size_t)k * n + i];
+        V3 r = pi - v3(pos_s[j]);
+        float l = length(r);
+        float dj = density[j];
+        fv = fv + (v3(vel_s[j]) - vi) * ((P.H - l) / dj);
+        float corr = (2.0f * P.rho0_d) / (di + dj);
+        fcurv = fcurv + (ni - v3(normal[j])) * corr;
+        fcoh = fcoh + (normalize(r) * st_kernel(l, P.H, P.st_term)) * corr;
+    }
+    fv = fv * ((P.k_visc * sqr(P.M)) * (P.visc_lapl / di));
+    fcoh = fcoh * (P.M * P.st_main);
+    V3 ft = ((-P.sigma) * P.M) * (fcurv + fcoh);
+    V3 fo = fv + ft;
+    f_other[i] = f4(fo, 0.0f);
+    // first predict_pos: the pressure force buffer is zero-filled at this
+    // point (pcisphsimluation.cpp:126)
+    V3 zero = {0.0f, 0.0f, 0.0f};
+    pos_pred[i] = f4(pci_predict(pi, vi, pci_accel(zero, fo, P), P), 1.0f);
+}
+
+__global__ __launch_bounds__(256) void k_update_pressure(int n, int iter, const float4* __restrict__ pos_pred,
+                                                         const int* __restrict__ nl, const int* __restrict__ nc,
+                                                         const int* __restrict__ bnl, const int* __restrict__ bnc,
+                                                         const float4* __restrict__ bpos_s,
+                                                         const float* __restrict__ bphi_s, int has_b, PciParams P,
+                                                         float* __restrict__ err, float* __restrict__ pressure,
+                                                         DevState* st) {
+    int done = st->done_iter;
+    if (done != 0 && done < iter) return;
+    int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    float dv = 0.0f;
+    if (i < n) {
+        V3 xi = v3(pos_pred[i]);
+        float pd = 0.0f, pdb = 0.0f;
+        int len = nc[i];
+        for (int k = 0; k < len; ++k) {
+            V3 r = xi - v3(pos_pred[nl[(size_t)k * n + i]]);
+            pd += cube(P.H2 - dot(r, r));
+        }
+        pd *= P.w_eval * P.M;
+        if (has_b) {
+            int blen = bnc[i];
+            for (int k = 0; k < blen; ++k) {
+                int j = bnl[(size_t)k * n + i];
+                V3 r = xi - v3(bpos_s[j]);
+                pdb += cube(P.H2 - dot(r, r)) * bphi_s[j];
+            }
+            pdb *= P.w_eval;
+        }
+        dv = cl_max(0.0f, (pd + pdb) - P.rho0_d);
+        err[i] = dv;
+        float p0 = (iter == 1) ? 0.0f : pressure[i];  // zero-filled per step (pcisphsimluation.cpp:118)
+        pressure[i] = p0 + dv * P.delta;
+    }
+    // dv >= +0, so its bits order like the values
+    uint32_t bits = __float_as_uint(dv);
+    for (int off = 32; off > 0; off >>= 1) {
+        uint32_t o = __shfl_xor(bits, off);
+        bits = o > bits ? o : bits;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(&st->maxerr[iter & (MAXERR_SLOTS - 1)], bits);
+}
+
+__global__ __launch_bounds__(256) void k_pressure_force(
+    int n, int iter, const float4* __restrict__ pos_s, const float4* __restrict__ vel_s,
+    const float* __restrict__ density, const float* __restrict__ pressure, const float4* __restrict__ f_other,
+    const int* __restrict__ nl, const int* __restrict__ nc, const int* __restrict__ bnl,
+    const int* __restrict__ bnc, const float4* __restrict__ bpos_s, const float* __restrict__ bphi_s, int has_b,
+    PciParams P, float4* __restrict__ f_press, float4* __restrict__ pos_pred, float4* __restrict__ pos_out,
+    float4* __restrict__ vel_out, DevState* st) {
+    int done = st->done_iter;
+    if (done != 0 && done < iter) return;
+    float mx = __uint_as_float(st->maxerr[iter & (MAXERR_SLOTS - 1)]);
+    bool last = iter >= P.max_iter || (iter >= P.min_iter && ((mx - P.rho0_host) / P.rho0_host < P.eta));
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (last) {
+            st->done_iter = iter;
+            st->iterations = iter;
+        }
+        st->maxerr[(iter + 1) & (MAXERR_SLOTS - 1)] = 0u;
+    }
+    int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    V3 xi = v3(pos_s[i]);
+    float di = density[i];
+    float C = pressure[i] / sqr(di);
+    V3 f = {0.0f, 0.0f, 0.0f}, fb = {0.0f, 0.0f, 0.0f};
+    int len = nc[i];
+    for (int k = 0; k < len; ++k) {
+        int j = nl[(size_t)k * n + i];
+        V3 r = xi - v3(pos_s[j]);
+        float l = length(r);
+        if (l >= 1e-5f) {
+            V3 gw = (r * (1.0f / l)) * sqr(P.H - l);
+            f = f + (C + pressure[j] / sqr(density[j])) * gw;
+        }
+    }
+    f = f * ((-sqr(P.M)) * P.press_grad);
+    if (has_b) {
+        int blen = bnc[i];
+        for (int k = 0; k < blen; ++k) {
+            int j = bnl[(size_t)k * n + i];
+            V3 r = xi - v3(bpos_s[j]);
+            float l = length(r);
+            V3 gw = (r * (1.0f / l)) * sqr(P.H - l);
+            fb = fb + (bphi_s[j] * (2.0f * C)) * gw;
+        }
+        fb = fb * ((-P.M) * P.press_grad);
+    }
+    V3 fp = f + fb;
+    f_press[i] = f4(fp, 0.0f);
+    V3 vi = v3(vel_s[i]);
+    V3 acc = pci_accel(fp, v3(f_other[i]), P);
+    if (!last) {
+        pos_pred[i] = f4(pci_predict(xi, vi, acc, P), 1.0f);
+    } else {
+        V3 x = xi, v = vi;
+        pci_integrate(x, v, acc, P);
+        pos_out[i] = f4(x, 1.0f);
+        vel_out[i] = f4(v, 0.0f);
+    }
+}
+
+// Final integration with a zero pressure force, for a loop of zero corrections.
+__global__ __launch_bounds__(256) void k_integrate_only(int n, const float4* __restrict__ pos_s,
+                                                        const float4* __restrict__ vel_s,
+                                                        const float4* __restrict__ f_other, PciParams P,
+                                                        float4* __restrict__ pos_out, float4* __restrict__ vel_out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    V3 x = v3(pos_s[i]), v = v3(vel_s[i]);
+    V3 zero = {0.0f, 0.0f, 0.0f};
+    pci_integrate(x, v, pci_accel(zero, v3(f_other[i]), P), P);
+    pos_out[i] = f4(x, 1.0f);
+    vel_out[i] = f4(v, 0.0f);
+}
+
+void launch_forces_predict(int n, const float4* pos_s, const float4* vel_s, const float* density,
+                           const float4* normal, const int* nl, const int* nc, const PciParams& P, float4* f_other,
+                           float4* pos_pred, hipStream_t s, Profiler* prof) {
+    if (n <= 0) return;
+    PROF_BEGIN(prof, "forces_predict", s);
+    hipLaunchKernelGGL(k_forces_predict, dim3((n + 255) / 256), dim3(256), 0, s, n, pos_s, vel_s, density, normal, nl,
+                       nc, P, f_other, pos_pred);
+    PROF_END(prof, s);
+}
+
+void launch_update_pressure(int n, int iter, const float4* pos_pred, const int* nl, const int* nc, const int* bnl,
+                            const int* bnc, const float4* bpos_s, const float* bphi_s, int has_b,
+                            const PciParams& P, float* err, float* pressure, DevState* st, hipStream_t s,
+                            Profiler* prof) {
+    if (n <= 0) return;
+    PROF_BEGIN(prof, "update_pressure", s);
+    hipLaunchKernelGGL(k_update_pressure, dim3((n + 255) / 256), dim3(256), 0, s, n, iter, pos_pred, nl, nc, bnl, bnc,
+                       bpos_s, bphi_s, has_b, P, err, pressure, st);
+    PROF_END(prof, s);
+}
+
+void launch_pressure_force(int n, int iter, const float4* pos_s, const float4* vel_s, const float* density,
+                           const float* pressure, const float4* f_other, const int* nl, const int* nc,
+                           const int* bnl, const int* bnc, const float4* bpos_s, const float* bphi_s, int has_b,
+                           const PciParams& P, float4* f_press, float4* pos_pred, float4* pos_out, float4* vel_out,
+                           DevState* st, hipStream_t s, Profiler* prof) {
+    if (n <= 0) return;
+    PROF_BEGIN(prof, "pressure_force", s);
+    hipLaunchKernelGGL(k_pressure_force, dim3((n + 255) / 256), dim3(256), 0, s, n, iter, pos_s, vel_s, density,
+                       pressure, f_other, nl, nc, bnl, bnc, bpos_s, bphi_s, has_b, P, f_press, pos_pred, pos_out,
+                       vel_out, st);
+    PROF_END(prof, s);
+}
+
+void launch_pci_integrate_only(int n, const float4* pos_s, const float4* vel_s, const float4* f_other,
+                               const PciParams& P, float4* pos_out, float4* vel_out, hipStream_t s,
+                               Profiler* prof) {
+    if (n <= 0) return;
+    PROF_BEGIN(prof, "integrate", s);
+    hipLaunchKernelGGL(k_integrate_only, dim3((n + 255) / 256), dim3(256), 0, s, n, pos_s, vel_s, f_other, P, pos_out,
+                       vel_out);
+    PROF_END(prof, s);
+}
+
+}  // namespace sph

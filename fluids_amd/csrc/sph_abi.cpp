@@ -1,0 +1,357 @@
+// extern "C" entry points of libsph (declared in include/sph.h). Every entry
+// point converts exceptions into status codes; the message of the last
+// failure of the calling thread is kept for sph_last_error().
+#include <cstring>
+#include <string>
+
+#include "solver.h"
+#include "sph.h"
+
+using namespace sph;
+
+struct sph_handle {
+    std::unique_ptr<SimulationBase> sim;
+};
+
+namespace {
+thread_local std::string g_last_error;
+
+template <class F> int guard(F&& f) {
+    try {
+        g_last_error.clear();
+        return f();
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return SPH_ERR_INVALID;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return SPH_ERR_INVALID;
+    }
+}
+
+Settings to_settings(const sph_settings* s) {
+    Settings o;
+    o.time_step = s->time_step;
+    o.max_vel = s->max_vel;
+    o.fluid_particle_radius = s->fluid_particle_radius;
+    o.fluid_support_radius = s->fluid_support_radius;
+    o.pcisph_max_iterations = s->pcisph_max_iterations;
+    o.pcisph_error_ratio = s->pcisph_error_ratio;
+    o.method = s->method;
+    o.rest_density = s->rest_density;
+    o.k_viscosity = s->k_viscosity;
+    o.gravity = s->gravity;
+    o.gas_stiffness = s->gas_stiffness;
+    o.surface_tension = s->surface_tension;
+    o.flags = s->flags;
+    return o;
+}
+
+void from_settings(const Settings& o, sph_settings* s) {
+    s->time_step = o.time_step;
+    s->max_vel = o.max_vel;
+    s->fluid_particle_radius = o.fluid_particle_radius;
+    s->fluid_support_radius = o.fluid_support_radius;
+    s->pcisph_max_iterations = o.pcisph_max_iterations;
+    s->pcisph_error_ratio = o.pcisph_error_ratio;
+    s->method = o.method;
+    s->rest_density = o.rest_density;
+    s->k_viscosity = o.k_viscosity;
+    s->gravity = o.gravity;
+    s->gas_stiffness = o.gas_stiffness;
+    s->surface_tension = o.surface_tension;
+    s->flags = o.flags;
+}
+
+void validate(const sph_settings* s) {
+    if (!s) throw Error(SPH_ERR_INVALID, "settings is NULL");
+    if (s->method != SPH_WCSPH && s->method != SPH_PCISPH) throw Error(SPH_ERR_INVALID, "Unknown simulation method!");
+    if (!(s->fluid_support_radius > 0.0f) || !(s->fluid_particle_radius > 0.0f))
+        throw Error(SPH_ERR_INVALID, "radii must be positive");
+    if (!(s->time_step > 0.0f)) throw Error(SPH_ERR_INVALID, "time_step must be positive");
+}
+
+SimulationBase& sim_of(sph_handle* h) {
+    if (!h || !h->sim) throw Error(SPH_ERR_INVALID, "handle is NULL");
+    return *h->sim;
+}
+
+std::vector<float4> to_f4(const float* xyzw, int n, float w) {
+    if (n < 0) throw Error(SPH_ERR_RANGE, "negative count");
+    if (n > 0 && !xyzw) throw Error(SPH_ERR_INVALID, "data is NULL");
+    std::vector<float4> v(n);
+    for (int i = 0; i < n; ++i) v[i] = make_float4(xyzw[4 * i], xyzw[4 * i + 1], xyzw[4 * i + 2], w);
+    return v;
+}
+
+int copy_samples(const std::vector<float4>& v, float* out, int cap) {
+    if (out)
+        for (int i = 0; i < (int)v.size() && i < cap; ++i) {
+            out[4 * i] = v[i].x;
+            out[4 * i + 1] = v[i].y;
+            out[4 * i + 2] = v[i].z;
+            out[4 * i + 3] = v[i].w;
+        }
+    return (int)v.size();
+}
+}  // namespace
+
+extern "C" {
+
+const char* sph_last_error(void) { return g_last_error.c_str(); }
+
+const char* sph_version(void) { return "sph-gfx950 0.1"; }
+
+int sph_default_settings(sph_settings* out) {
+    return guard([&] {
+        if (!out) throw Error(SPH_ERR_INVALID, "out is NULL");
+        from_settings(Settings{}, out);
+        return SPH_OK;
+    });
+}
+
+int sph_load_settings(const char* path, sph_settings* out) {
+    return guard([&] {
+        if (!path || !out) throw Error(SPH_ERR_INVALID, "NULL argument");
+        from_settings(load_settings(path), out);
+        return SPH_OK;
+    });
+}
+
+sph_handle* sph_create(const sph_settings* settings) {
+    sph_handle* h = nullptr;
+    int rc = guard([&] {
+        validate(settings);
+        auto sim = build_simulation(to_settings(settings));
+        h = new sph_handle{std::move(sim)};
+        return SPH_OK;
+    });
+    return rc == SPH_OK ? h : nullptr;
+}
+
+int sph_destroy(sph_handle* h) {
+    return guard([&] {
+        delete h;
+        return SPH_OK;
+    });
+}
+
+int sph_reset(sph_handle* h, const sph_settings* settings) {
+    return guard([&] {
+        validate(settings);
+        sim_of(h).reset(to_settings(settings));
+        return SPH_OK;
+    });
+}
+
+int sph_set_rect_limits(sph_handle* h, float width, float height, float depth) {
+    return guard([&] {
+        sim_of(h).set_rect_limits(width, height, depth);
+        return SPH_OK;
+    });
+}
+
+int sph_add_volume(sph_handle* h, const float* xyzw, int n) {
+    return guard([&] {
+        auto v = to_f4(xyzw, n, 1.0f);
+        sim_of(h).add_volume(std::make_shared<FixedVolume>(std::move(v)));
+        return SPH_OK;
+    });
+}
+
+int sph_add_box_volume(sph_handle* h, const float size[3], const float center[3]) {
+    return guard([&] {
+        if (!size || !center) throw Error(SPH_ERR_INVALID, "NULL argument");
+        sim_of(h).add_volume(std::make_shared<BoxVolume>(size, center));
+        return SPH_OK;
+    });
+}
+
+int sph_add_boundary(sph_handle* h, const float* local_xyzw, int n, int can_move, float mass, const float R[9],
+                     const float com[3]) {
+    return guard([&] {
+        auto v = to_f4(local_xyzw, n, 1.0f);
+        return sim_of(h).add_boundary(v, can_move != 0, mass, R, com);
+    });
+}
+
+int sph_set_body_state(sph_handle* h, int body, const float R[9], const float com[3], const float lin_vel[3],
+                       const float ang_vel[3]) {
+    return guard([&] {
+        sim_of(h).set_body_state(body, R, com, lin_vel, ang_vel);
+        return SPH_OK;
+    });
+}
+
+int sph_get_body_wrench(sph_handle* h, int body, float force[3], float torque[3]) {
+    return guard([&] {
+        float f[3], t[3];
+        sim_of(h).body_wrench(body, f, t);
+        for (int k = 0; k < 3; ++k) {
+            if (force) force[k] = f[k];
+            if (torque) torque[k] = t[k];
+        }
+        return SPH_OK;
+    });
+}
+
+int sph_step(sph_handle* h) {
+    return guard([&] {
+        sim_of(h).simulate();
+        return SPH_OK;
+    });
+}
+
+int sph_step_n(sph_handle* h, int n) {
+    return guard([&] {
+        if (n < 0) throw Error(SPH_ERR_RANGE, "negative step count");
+        auto& s = sim_of(h);
+        for (int k = 0; k < n; ++k) s.simulate();
+        return SPH_OK;
+    });
+}
+
+int sph_step_dt(sph_handle* h, float dt) {
+    return guard([&] {
+        if (!(dt > 0.0f)) throw Error(SPH_ERR_INVALID, "dt must be positive");
+        sim_of(h).step_dt(dt);
+        return SPH_OK;
+    });
+}
+
+int sph_synchronize(sph_handle* h) {
+    return guard([&] {
+        sim_of(h).synchronize();
+        return SPH_OK;
+    });
+}
+
+int sph_particle_count(sph_handle* h) {
+    return guard([&] { return sim_of(h).particle_count(); });
+}
+
+int sph_boundary_particle_count(sph_handle* h) {
+    return guard([&] { return sim_of(h).boundary_particle_count(); });
+}
+
+int sph_get_positions(sph_handle* h, float* xyzw, int n) {
+    return guard([&] {
+        sim_of(h).get_state(0, xyzw, n);
+        return SPH_OK;
+    });
+}
+int sph_set_positions(sph_handle* h, const float* xyzw, int n) {
+    return guard([&] {
+        sim_of(h).set_state(0, xyzw, n);
+        return SPH_OK;
+    });
+}
+int sph_get_velocities(sph_handle* h, float* xyzw, int n) {
+    return guard([&] {
+        sim_of(h).get_state(1, xyzw, n);
+        return SPH_OK;
+    });
+}
+int sph_set_velocities(sph_handle* h, const float* xyzw, int n) {
+    return guard([&] {
+        sim_of(h).set_state(1, xyzw, n);
+        return SPH_OK;
+    });
+}
+int sph_get_half_velocities(sph_handle* h, float* xyzw, int n) {
+    return guard([&] {
+        sim_of(h).get_state(2, xyzw, n);
+        return SPH_OK;
+    });
+}
+int sph_set_half_velocities(sph_handle* h, const float* xyzw, int n) {
+    return guard([&] {
+        sim_of(h).set_state(2, xyzw, n);
+        return SPH_OK;
+    });
+}
+
+int sph_get_field(sph_handle* h, int field, void* out, long long cap) {
+    return guard([&] { return (int)sim_of(h).get_field(field, out, cap); });
+}
+
+int sph_get_step_stats(sph_handle* h, sph_step_stats* out) {
+    return guard([&] {
+        if (!out) throw Error(SPH_ERR_INVALID, "out is NULL");
+        sim_of(h).stats(*out);
+        return SPH_OK;
+    });
+}
+
+int sph_device_state(sph_handle* h, void** pos, void** vel, void** vel_half) {
+    return guard([&] {
+        auto& s = sim_of(h);
+        if (pos) *pos = s.d_pos.get();
+        if (vel) *vel = s.d_vel.get();
+        if (vel_half) *vel_half = s.d_vh.get();
+        return SPH_OK;
+    });
+}
+
+void* sph_stream(sph_handle* h) { return (h && h->sim) ? (void*)h->sim->stream() : nullptr; }
+
+int sph_set_profiling(sph_handle* h, int on) {
+    return guard([&] {
+        sim_of(h).profiler().enabled = on != 0;
+        return SPH_OK;
+    });
+}
+
+int sph_get_kernel_times(sph_handle* h, char* names, int* launches, double* total_ms, int cap) {
+    return guard([&] {
+        auto& p = sim_of(h).profiler();
+        p.resolve();
+        int n = (int)p.entries.size();
+        for (int k = 0; k < n && k < cap; ++k) {
+            if (names) {
+                std::strncpy(names + 48 * k, p.entries[k].name.c_str(), 47);
+                names[48 * k + 47] = 0;
+            }
+            if (launches) launches[k] = (int)p.entries[k].launches;
+            if (total_ms) total_ms[k] = p.entries[k].total_ms;
+        }
+        return n;
+    });
+}
+
+int sph_reset_kernel_times(sph_handle* h) {
+    return guard([&] {
+        sim_of(h).profiler().reset();
+        return SPH_OK;
+    });
+}
+
+int sph_sample_box_volume(const float size[3], const float center[3], float radius, float* out, int cap) {
+    return guard([&] {
+        if (!size || !center) throw Error(SPH_ERR_INVALID, "NULL argument");
+        return copy_samples(sample_box_volume(size, center, radius), out, cap);
+    });
+}
+int sph_sample_fishtank(float width, float height, float depth, float spacing, float* out, int cap) {
+    return guard([&] {
+        if (!(spacing > 0.0f)) throw Error(SPH_ERR_INVALID, "spacing must be positive");
+        return copy_samples(sample_fishtank(width, height, depth, spacing), out, cap);
+    });
+}
+int sph_sample_cube(float side, float spacing, float* out, int cap) {
+    return guard([&] {
+        if (!(spacing > 0.0f)) throw Error(SPH_ERR_INVALID, "spacing must be positive");
+        return copy_samples(sample_cube(side, spacing), out, cap);
+    });
+}
+int sph_sample_wall(float width, float height, float spacing, float* out, int cap) {
+    return guard([&] {
+        if (!(spacing > 0.0f)) throw Error(SPH_ERR_INVALID, "spacing must be positive");
+        return copy_samples(sample_wall(width, height, spacing), out, cap);
+    });
+}
+
+}  // extern "C"

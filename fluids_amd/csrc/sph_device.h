@@ -1,0 +1,165 @@
+// Device helpers of the gfx950 SPH kernels.
+//
+// Arithmetic is IEEE fp32 with no contraction (the library is compiled with
+// -ffp-contract=off and correctly rounded division and square root), and every
+// expression keeps the association of the reference OpenCL expression it
+// restates, so a kernel produces the same bits as any other IEEE evaluation of
+// that expression. Built-in semantics are those of OpenCL 1.2:
+//   max(x,y) = x < y ? y : x      min(x,y) = y < x ? y : x
+//   clamp(x,lo,hi) = min(max(x,lo),hi)
+//   pown(x,2) = x*x, pown(x,3) = (x*x)*x, pown(x,7) left-to-right products
+//   fast_length(r) = length(r) = sqrt(dot(r,r))
+//   normalize(r) = fast_normalize(r) = r / length(r), and r itself for r == 0
+// The w component is left out of all arithmetic: positions carry w == 1 and
+// velocities, forces and normals w == 0 on the whole path
+// (boxvolume.cpp:71, boundaryhandler.cpp:132, pcisph.cl:279-280), so every w
+// term of the reference's float4 dot products is exactly +0.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sph {
+
+constexpr int LIST_CAP = 50;      // NEIGH_LIST_MAX_LENGTH (kernels/common.h:4)
+constexpr int MAXERR_SLOTS = 16;  // ring of per-correction max density errors
+
+// Uniform grid restated from kernels/grid.h and fluid/simulation/grid.cpp:61-68.
+// Cell ranges live in a dense box: all cps cells in y (the 5 m grid is
+// periodic and the integrators allow y up to 10 m), [bx0, bx0+bnx) in x and
+// [bz0, bz0+bnz) in z. Entries are int4 {start, epoch, end, 0}; an entry whose
+// epoch differs from the current build reads as the empty range [0, 0), which
+// is what the reference's zero-filled int2[cps^3] holds for an empty cell.
+struct GridDev {
+    float half;         // size / 2 = 2.5f
+    float cell;         // cell size: full fp32 support radius
+    float sqr_support;  // CELL_SIZE*CELL_SIZE with the 6-decimal literal (grid.cl:3)
+    int cps;            // cells per side
+    long long cells;    // cps^3
+    int bx0, bz0, bnx, bnz;
+};
+
+// Per-step device state written by the kernels.
+struct DevState {
+    unsigned int maxerr[MAXERR_SLOTS];  // float bits of max density error per correction (ring)
+    int done_iter;                      // correction at which the loop ended (0: running)
+    int iterations;
+    int out_of_grid;
+    int edge_reads;
+    int nonfinite;                      // non-finite positions met by the hash kernel
+    int pad[3];
+};
+
+__device__ __forceinline__ float cl_max(float a, float b) { return (a < b) ? b : a; }
+__device__ __forceinline__ float cl_min(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float cl_clamp(float x, float lo, float hi) { return cl_min(cl_max(x, lo), hi); }
+__device__ __forceinline__ float cl_sign(float x) {
+    if (x > 0.0f) return 1.0f;
+    if (x < 0.0f) return -1.0f;
+    if (x != x) return 0.0f;
+    return x;
+}
+__device__ __forceinline__ float sqr(float x) { return x * x; }
+__device__ __forceinline__ float cube(float x) { return (x * x) * x; }
+__device__ __forceinline__ float pown7(float x) {
+    float r = x * x;
+    r = r * x; r = r * x; r = r * x; r = r * x; r = r * x;
+    return r;
+}
+
+struct V3 { float x, y, z; };
+__device__ __forceinline__ V3 v3(float4 a) { return {a.x, a.y, a.z}; }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 operator*(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V3 operator*(float s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ V3 operator/(V3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ float length(V3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ V3 normalize(V3 a) {
+    float d = dot(a, a);
+    if (d == 0.0f) return a;
+    float l = sqrtf(d);
+    return {a.x / l, a.y / l, a.z / l};
+}
+__device__ __forceinline__ V3 clamp3(V3 a, float lo, float hi) {
+    return {cl_clamp(a.x, lo, hi), cl_clamp(a.y, lo, hi), cl_clamp(a.z, lo, hi)};
+}
+__device__ __forceinline__ float4 f4(V3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+
+// get_grid_coordinates (kernels/grid.h:74-91)
+__device__ __forceinline__ void grid_coords(float4 p, const GridDev& g, int& cx, int& cy, int& cz) {
+    cx = (int)floorf((p.x + g.half) / g.cell);
+    cy = (int)floorf((p.y + g.half) / g.cell);
+    cz = (int)floorf((p.z + g.half) / g.cell);
+    cx = (cx + g.cps) % g.cps;
+    cy = (cy + g.cps) % g.cps;
+    cz = (cz + g.cps) % g.cps;
+}
+
+// split_by_3 / morton_3d_encode with OpenCL 32-bit semantics (kernels/morton.h:37-56):
+// x << 32 on a 32-bit uint shifts by 0, and the 64-bit masks keep their low 32 bits.
+__device__ __forceinline__ uint32_t split_by_3(uint32_t a) {
+    uint32_t x = a & 0xffffu;
+    x = (x | (x << 16)) & 0xff0000ffu;
+    x = (x | (x << 8)) & 0x0f00f00fu;
+    x = (x | (x << 4)) & 0xc30c30c3u;
+    x = (x | (x << 2)) & 0x49249249u;
+    return x;
+}
+__device__ __forceinline__ uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+    return split_by_3(x) | (split_by_3(y) << 1) | (split_by_3(z) << 2);
+}
+
+// Dense-box index of an in-grid cell, or -1 when it lies outside the box.
+__device__ __forceinline__ long long box_index(int x, int y, int z, const GridDev& g) {
+    int lx = x - g.bx0, lz = z - g.bz0;
+    if ((unsigned)lx >= (unsigned)g.bnx || (unsigned)lz >= (unsigned)g.bnz) return -1;
+    return ((long long)lz * g.cps + y) * g.bnx + lx;
+}
+
+// cell_interval[CELL_ID(x, y, z)] for a neighbour coordinate that may lie one
+// cell outside [0, cps) (the reference does not wrap neighbour offsets,
+// grid.cl:165-166). CELL_ID is linear, so such a coordinate names another
+// in-range cell unless the linear id leaves [0, cps^3), where the reference
+// reads out of bounds; that case reads as empty and is counted.
+__device__ __forceinline__ int2 cell_range(const int4* __restrict__ R, int epoch, int x, int y, int z,
+                                           const GridDev& g, int& edge) {
+    int c = g.cps;
+    if ((unsigned)x >= (unsigned)c || (unsigned)y >= (unsigned)c || (unsigned)z >= (unsigned)c) {
+        long long L = (long long)c * ((long long)c * z + y) + x;
+        if (L < 0 || L >= g.cells) {
+            ++edge;
+            return make_int2(0, 0);
+        }
+        long long cc = (long long)c * c;
+        z = (int)(L / cc);
+        y = (int)((L / c) % c);
+        x = (int)(L % c);
+    }
+    long long b = box_index(x, y, z, g);
+    if (b < 0) return make_int2(0, 0);
+    int4 e = R[b];
+    return (e.y == epoch) ? make_int2(e.x, e.z) : make_int2(0, 0);
+}
+
+// CELL_NEIGH_OFFSET (kernels/grid.h:15-42): own cell, then the 26 others in
+// lexicographic (dx, dy, dz) order.
+__device__ __forceinline__ void neigh_offset(int o, int& dx, int& dy, int& dz) {
+    if (o == 0) { dx = dy = dz = 0; return; }
+    int k = o - 1;
+    if (k >= 13) ++k;  // skip (0,0,0), which is lexicographic position 13
+    dx = k / 9 - 1;
+    dy = (k / 3) % 3 - 1;
+    dz = k % 3 - 1;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; remap so that each
+// XCD owns a contiguous run of particles (Morton-adjacent neighbours then
+// share that XCD's L2). Bijective on [0, nb).
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    int q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+}  // namespace sph

@@ -1,0 +1,112 @@
+// WCSPH kernels on gfx950, restating kernels/wcsph.cl. compute_acceleration
+// (wcsph.cl:81-173) and predict_vel_n_pos (wcsph.cl:176-224) are fused: the
+// integrator reads only the particle's own acceleration, so one pass computes
+// the acceleration, stores it and advances the particle. Density, Tait
+// pressure and the lists come from k_lists_density<true> (grid.hip), the
+// normals from k_normals.
+#include "kernels.h"
+#include "sph_device.h"
+
+namespace sph {
+
+__device__ __forceinline__ float st_kernel_wc(float r, float h, float c) {
+    // st_kernel (kernels/surface_tension.h:23-36)
+    if (r <= 0.5f * h) {
+        float t1 = h - r;
+        float t2 = cube(t1) * cube(r);
+        return 2.0f * t2 - c;
+    } else if (r <= h) {
+        float t1 = h - r;
+        return cube(t1) * cube(r);
+    }
+    return 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_accel_integrate(
+    int n, const float4* __restrict__ pos_s, const float4* __restrict__ vel_s, const float4* __restrict__ vh_s,
+    const float* __restrict__ density, const float* __restrict__ pressure, const float4* __restrict__ normal,
+    const int* __restrict__ nl, const int* __restrict__ nc, const int* __restrict__ bnl,
+    const int* __restrict__ bnc, const float4* __restrict__ bpos_s, const float* __restrict__ bphi_s, WcParams P,
+    float4* __restrict__ accel, float4* __restrict__ pos_out, float4* __restrict__ vel_out,
+    float4* __restrict__ vh_out) {
+    int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 p4 = pos_s[i];
+    V3 pi = v3(p4), vi = v3(vel_s[i]);
+    float di = density[i], pri = pressure[i];
+    V3 ni = v3(normal[i]);
+    float C = pri / sqr(di);
+    const float h = P.h;
+    V3 bacc = {0.0f, 0.0f, 0.0f};
+    int blen = bnc[i];
+    for (int k = 0; k < blen; ++k) {
+        int j = bnl[(size_t)k * n + i];
+        V3 r = pi - v3(bpos_s[j]);
+        float rn = length(r);
+        V3 t = (-bphi_s[j]) * r;
+        t = t * (P.spiky_grad / rn);
+        t = t * sqr(h - rn);
+        t = t * 2.0f;
+        t = t * C;
+        bacc = bacc + t;
+    }
+    V3 pacc = {0.0f, 0.0f, 0.0f}, vacc = {0.0f, 0.0f, 0.0f}, fcurv = {0.0f, 0.0f, 0.0f}, fcoh = {0.0f, 0.0f, 0.0f};
+    int len = nc[i];
+    for (int k = 0; k < len; ++k) {
+        int j = nl[(size_t)k * n + i];
+        float dj = density[j], prj = pressure[j];
+        V3 r = pi - v3(pos_s[j]);
+        float rn = length(r);
+        // wcsph.cl:152 compares against the double literal 0.00001
+        if (di > 0.0f && dj > 0.0f && (double)rn > 0.00001) {
+            pacc = pacc + ((r * (P.spiky_grad / rn)) * sqr(h - rn)) * (C + prj / sqr(dj));
+            vacc = vacc + ((v3(vel_s[j]) - vi) * (h - rn)) / dj;
+            float corr = (2.0f * P.rho0) / (di + dj);
+            fcurv = fcurv + (ni - v3(normal[j])) * corr;
+            fcoh = fcoh + (normalize(r) * st_kernel_wc(rn, h, P.st_term)) * corr;
+        }
+    }
+    pacc = pacc * (-P.M);
+    vacc = vacc * (P.visc_lapl * P.k_visc);
+    fcoh = fcoh * P.st_main;
+    V3 ast = (-P.sigma) * (fcurv + fcoh);
+    V3 g = {P.gx, P.gy, P.gz};
+    V3 a = (((g + ast) + pacc) + vacc) + bacc;
+    accel[i] = f4(a, 0.0f);
+
+    // predict_vel_n_pos (wcsph.cl:194-223), the leap-frog with the reference's
+    // double use of dt*acc (Q12)
+    V3 x = pi;
+    V3 vh = v3(vh_s[i]);
+    vh = vh + P.dt * a;
+    x = x + P.dt * clamp3(vh + P.dt * a, -P.max_vel, P.max_vel);
+    float vn = length(vh);
+    V3 cp = {cl_clamp(x.x, P.lo_x, P.hi_x), cl_clamp(x.y, P.lo_y, 10.0f), cl_clamp(x.z, P.lo_z, P.hi_z)};
+    V3 dvec = cp - x;
+    float d = length(dvec);
+    if (d > 0.0f && vn > 0.0f) {
+        V3 nrm = normalize(V3{cl_sign(dvec.x), cl_sign(dvec.y), cl_sign(dvec.z)});
+        float rest = (1.0f * d) / (P.dt * vn);
+        float kk = (1.0f + rest) * dot(vh, nrm);
+        vh = vh - kk * nrm;
+        x = cp;
+    }
+    V3 vt = vh + (P.dt * a) * 0.5f;
+    pos_out[i] = f4(x, p4.w);
+    vh_out[i] = f4(clamp3(vh, -P.max_vel, P.max_vel), 0.0f);
+    vel_out[i] = f4(clamp3(vt, -P.max_vel, P.max_vel), 0.0f);
+}
+
+void launch_accel_integrate(int n, const float4* pos_s, const float4* vel_s, const float4* vh_s,
+                            const float* density, const float* pressure, const float4* normal, const int* nl,
+                            const int* nc, const int* bnl, const int* bnc, const float4* bpos_s,
+                            const float* bphi_s, const WcParams& P, float4* accel, float4* pos_out,
+                            float4* vel_out, float4* vh_out, hipStream_t s, Profiler* prof) {
+    if (n <= 0) return;
+    PROF_BEGIN(prof, "accel_integrate", s);
+    hipLaunchKernelGGL(k_accel_integrate, dim3((n + 255) / 256), dim3(256), 0, s, n, pos_s, vel_s, vh_s, density,
+                       pressure, normal, nl, nc, bnl, bnc, bpos_s, bphi_s, P, accel, pos_out, vel_out, vh_out);
+    PROF_END(prof, s);
+}
+
+}  // namespace sph

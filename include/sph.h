@@ -1,0 +1,257 @@
+/*
+ * sph.h — C ABI of libsph, the MI355X (gfx950) SPH fluid solver.
+ *
+ * The solver replaces the OpenCL simulation path of paivett/fluids: the
+ * abstract C++ class FluidSimulation (fluid/simulation/fluidsimulation.h:23-92)
+ * with its two implementations PCISPHSimulation and WCSPHSimulation, the
+ * uniform Grid, the BoundaryHandler and the OpenCL kernels under kernels/.
+ * Every entry point is extern "C", takes plain pointers and sizes, returns an
+ * int status (SPH_OK == 0, negative on error) unless stated otherwise, and
+ * never lets a C++ exception cross the ABI; sph_last_error() gives the
+ * message of the calling thread's last failure.
+ *
+ * Threading: a handle owns one HIP stream and its device buffers. A handle is
+ * not thread-safe; different handles are independent (no global state besides
+ * the per-thread error string).
+ *
+ * Particle order: the reference integrates into its "unsorted" buffers at the
+ * sorted index (pcisphsimluation.cpp:160-162), so the stored order after a
+ * step is the previous step's Morton order. sph_get_positions and friends
+ * return that stored order, exactly as the reference's VBO holds it.
+ */
+#ifndef SPH_H_
+#define SPH_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPH_OK 0
+#define SPH_ERR_INVALID (-1)   /* bad argument */
+#define SPH_ERR_HIP (-2)       /* HIP runtime failure (message holds the HIP error) */
+#define SPH_ERR_STATE (-3)     /* call not valid in the current state */
+#define SPH_ERR_RANGE (-4)     /* size or index out of range */
+#define SPH_ERR_PARSE (-5)     /* configuration file could not be parsed */
+
+/* SimulationSettings::Method (settings/simulationsettings.h:28-31) */
+#define SPH_WCSPH 0
+#define SPH_PCISPH 1
+
+/* sph_settings.flags. 0 reproduces the reference exactly. */
+#define SPH_FIX_WCSPH_DENSITY 1  /* WCSPH density evaluates W_POLY6 on r^2 instead of r (wcsph.cl:61-62,70-71) */
+
+/*
+ * PhysicsSettings (settings/physicssettings.h:5-25) and SimulationSettings
+ * (settings/simulationsettings.h:9-39) merged into one plain struct; the keys
+ * are those of config.pci / config.wc.
+ */
+typedef struct sph_settings {
+    float time_step;
+    float max_vel;
+    float fluid_particle_radius;
+    float fluid_support_radius;
+    int pcisph_max_iterations;
+    float pcisph_error_ratio;
+    int method; /* SPH_WCSPH or SPH_PCISPH */
+    float rest_density;
+    float k_viscosity;
+    float gravity;
+    float gas_stiffness;
+    float surface_tension;
+    int flags; /* SPH_FIX_* bits; 0 = reference behaviour */
+} sph_settings;
+
+/* Per-step diagnostics (no counterpart in the reference, whose failure modes
+ * are silent: list truncation at 50, grid wrap, out-of-range cell reads). */
+typedef struct sph_step_stats {
+    int iterations;               /* PCISPH corrections run in the last step */
+    int fluid_list_overflows;     /* particles whose fluid list hit the cap of 50 */
+    int boundary_list_overflows;  /* particles whose boundary list hit the cap of 50 */
+    int edge_reads;               /* neighbour-cell ids outside [0, cps^3) (undefined reads in the reference) */
+    int out_of_grid;              /* particles whose cell lies outside the allocated range box */
+    int nan_positions;            /* particles with a non-finite position after the step */
+    long long fluid_neighbours;   /* sum of fluid list lengths */
+    long long boundary_neighbours;/* sum of boundary list lengths */
+    float max_density_error;      /* max(rho* - rho0, 0) of the last correction */
+} sph_step_stats;
+
+typedef struct sph_handle sph_handle;
+
+/* Message of the calling thread's last failure ("" if none). */
+const char* sph_last_error(void);
+
+/* Library version string. */
+const char* sph_version(void);
+
+/* ---------------------------------------------------------------- settings */
+
+/* Reference defaults of PhysicsSettings/SimulationSettings
+ * (physicssettings.h:17-22, simulationsettings.h:33-39). */
+int sph_default_settings(sph_settings* out);
+
+/* Settings::load (settings/settings.cpp:14-59, settings/fileparser.cpp:7-50):
+ * key=value file, first whitespace token of each side, every key required,
+ * unknown keys ignored, method must be wcsph|pcisph and render_method
+ * particles|screenspace. Returns SPH_ERR_PARSE with the reference's message
+ * on a missing file or key. */
+int sph_load_settings(const char* path, sph_settings* out);
+
+/* ------------------------------------------------------------ construction */
+
+/* FluidSimulationFactory::build_simulation (fluidsimulationfactory.h:13-40):
+ * constructs the solver selected by settings->method. Returns NULL on error. */
+sph_handle* sph_create(const sph_settings* settings);
+
+/* FluidSimulation::~FluidSimulation (fluidsimulation.h:29). */
+int sph_destroy(sph_handle* h);
+
+/* FluidSimulation::reset (fluidsimulation.h:47-48; pcisphsimluation.cpp:257-266,
+ * wcsphsimluation.cpp:205-214): new parameters, boundary psi recomputed,
+ * fluid re-initialised from the stored volumes (PCISPH runs its relaxation
+ * step). */
+int sph_reset(sph_handle* h, const sph_settings* settings);
+
+/* FluidSimulation::set_rect_limits (fluidsimulation.h:79;
+ * pcisphsimluation.cpp:557-564): the clamp box of the integrators. */
+int sph_set_rect_limits(sph_handle* h, float width, float height, float depth);
+
+/* FluidSimulation::add_volume (fluidsimulation.h:91; pcisphsimluation.cpp:525-530)
+ * with a FluidVolume whose particles() (fluidvolume.h:10) returns this fixed
+ * set. Re-initialises the fluid from all volumes; PCISPH then runs one
+ * relaxation step and zeroes the velocities (pcisphsimluation.cpp:548-550). */
+int sph_add_volume(sph_handle* h, const float* xyzw, int n);
+
+/* FluidSimulation::add_volume with a BoxVolume (fluid/simulation/boxvolume.cpp:11-40),
+ * re-sampled from the particle radius on every (re)initialisation. */
+int sph_add_box_volume(sph_handle* h, const float size[3], const float center[3]);
+
+/* FluidSimulation::add_boundary (fluidsimulation.h:66-68;
+ * boundaryhandler.cpp:95-153) for a RigidBody whose surface_sampling()
+ * (rigidbody.h:104) returned local_xyzw. R (row-major basis rows, as
+ * btTransform::getBasis) and com give the body's transform at sampling time
+ * (boundaryhandler.cpp:259-274). Only bodies with mass > 0 receive a fluid
+ * wrench (boundaryhandler.cpp:322). Returns the body id (>= 0) or an error. */
+int sph_add_boundary(sph_handle* h, const float* local_xyzw, int n, int can_move, float mass,
+                     const float R[9], const float com[3]);
+
+/* Replaces the RigidBody reads of BoundaryHandler::sync (boundaryhandler.cpp:259-270:
+ * transform(), angular_vel(), linear_vel(), position()) for the next step. Any
+ * pointer may be NULL to keep the previous value. */
+int sph_set_body_state(sph_handle* h, int body, const float R[9], const float com[3],
+                       const float lin_vel[3], const float ang_vel[3]);
+
+/* Replaces RigidBody::apply_force_n_torque (rigidbody.cpp:49-54): the summed
+ * fluid force and torque (about com) of the last step (clsum of
+ * boundaryhandler.cpp:338-346). Zero for bodies with mass <= 0. */
+int sph_get_body_wrench(sph_handle* h, int body, float force[3], float torque[3]);
+
+/* ------------------------------------------------------------------- step */
+
+/* FluidSimulation::simulate (fluidsimulation.h:36): one step with the
+ * settings' time step (PCISPHSimulation::_simulate_pcisph_step,
+ * pcisphsimluation.cpp:72-165; WCSPHSimulation::simulate,
+ * wcsphsimluation.cpp:40-106). Asynchronous on the handle's stream. */
+int sph_step(sph_handle* h);
+
+/* sph_step repeated n times (no host synchronisation in between). */
+int sph_step_n(sph_handle* h, int n);
+
+/* step(dt): sets the time step (recomputing the PCISPH scale factor, which
+ * depends on it: pcisphsimluation.cpp:432) and runs one step. */
+int sph_step_dt(sph_handle* h, float dt);
+
+/* Waits for the handle's stream. */
+int sph_synchronize(sph_handle* h);
+
+/* FluidSimulation::particle_count / boundary_particle_count
+ * (fluidsimulation.h:54,84). Return the count (>= 0) or an error. */
+int sph_particle_count(sph_handle* h);
+int sph_boundary_particle_count(sph_handle* h);
+
+/* --------------------------------------------------------- buffers (host) */
+
+/* Positions/velocities in the stored order (xyzw, n = particle_count). These
+ * replace the reference's GL VBO (scene/fluid.cpp:27) and its device-only
+ * velocity buffers. sph_set_* write w = 1 for positions and w = 0 for
+ * velocities, which is what every producer of the reference writes. */
+int sph_get_positions(sph_handle* h, float* xyzw, int n);
+int sph_set_positions(sph_handle* h, const float* xyzw, int n);
+int sph_get_velocities(sph_handle* h, float* xyzw, int n);
+int sph_set_velocities(sph_handle* h, const float* xyzw, int n);
+/* WCSPH half-step velocities (wcsphsimluation.cpp:140). */
+int sph_get_half_velocities(sph_handle* h, float* xyzw, int n);
+int sph_set_half_velocities(sph_handle* h, const float* xyzw, int n);
+
+/* Field ids for sph_get_field. Fields are left by the last step, in sorted
+ * order unless stated; vector fields are 4 floats per element. */
+#define SPH_FIELD_KEYS 0            /* u32[N]  sorted Morton cell keys (grid.cl:35) */
+#define SPH_FIELD_PERM 1            /* i32[N]  sort permutation (the reference's mask) */
+#define SPH_FIELD_NCOUNT 2          /* i32[N]  fluid neighbour counts */
+#define SPH_FIELD_NLIST 3           /* i32[50N] fluid lists, column-major list[k*N+i], unused slots -1 */
+#define SPH_FIELD_BNCOUNT 4         /* i32[N]  boundary neighbour counts */
+#define SPH_FIELD_BNLIST 5          /* i32[50N] boundary lists */
+#define SPH_FIELD_POS_SORTED 6      /* f4[N] */
+#define SPH_FIELD_VEL_SORTED 7      /* f4[N] */
+#define SPH_FIELD_DENSITY 8         /* f32[N]  initial density (PCISPH) / density (WCSPH) */
+#define SPH_FIELD_PRESSURE 9        /* f32[N]  accumulated pressure (PCISPH) / Tait pressure (WCSPH) */
+#define SPH_FIELD_ERR 10            /* f32[N]  density variation of the last correction */
+#define SPH_FIELD_NORMAL 11         /* f4[N] */
+#define SPH_FIELD_FORCE_OTHER 12    /* f4[N]  viscosity + surface tension force */
+#define SPH_FIELD_FORCE_PRESSURE 13 /* f4[N]  pressure force of the last correction */
+#define SPH_FIELD_POS_PRED 14       /* f4[N]  predicted positions of the last correction */
+#define SPH_FIELD_ACCEL 15          /* f4[N]  WCSPH acceleration */
+#define SPH_FIELD_POS 16            /* f4[N]  stored-order positions */
+#define SPH_FIELD_VEL 17            /* f4[N]  stored-order velocities */
+#define SPH_FIELD_VEL_HALF 18       /* f4[N]  stored-order WCSPH half-step velocities */
+#define SPH_FIELD_BPOS_SORTED 19    /* f4[B]  sorted boundary positions */
+#define SPH_FIELD_BPHI_SORTED 20    /* f32[B] sorted boundary psi */
+#define SPH_FIELD_BKEYS 21          /* u32[B] sorted boundary keys */
+#define SPH_FIELD_BPERM 22          /* i32[B] boundary sort permutation */
+#define SPH_FIELD_BPHI 23           /* f32[B] psi in sampling order */
+#define SPH_FIELD_BFORCE 24         /* f4[B]  per-boundary-particle fluid force (bodies with mass > 0) */
+#define SPH_FIELD_BTORQUE 25        /* f4[B]  per-boundary-particle torque */
+#define SPH_FIELD_RANGES 26         /* i32[3*C] (reference cell id, start, end) of occupied fluid cells, by start */
+#define SPH_FIELD_BRANGES 27        /* i32[3*C] same for the boundary grid */
+#define SPH_FIELD_VEL_HALF_SORTED 28/* f4[N] */
+#define SPH_FIELD_COUNT 29
+
+/* Copies field `field` into out (capacity `cap` elements of the field's
+ * element type; pass out == NULL to query the element count). Synchronises
+ * the stream. Returns the element count or an error. */
+int sph_get_field(sph_handle* h, int field, void* out, long long cap);
+
+/* Diagnostics of the last step (synchronises). */
+int sph_get_step_stats(sph_handle* h, sph_step_stats* out);
+
+/* Device pointers of the stored-order buffers (float4[N]) for zero-copy use,
+ * valid until the next call that changes the particle count. */
+int sph_device_state(sph_handle* h, void** pos, void** vel, void** vel_half);
+/* The handle's HIP stream (hipStream_t). */
+void* sph_stream(sph_handle* h);
+
+/* ------------------------------------------------------------- profiling */
+
+/* Per-kernel HIP-event timing on the handle's stream (off by default). */
+int sph_set_profiling(sph_handle* h, int on);
+/* Fills names (a buffer of `cap` * 48 chars), launches and total milliseconds
+ * per kernel name; returns the number of kernel names. */
+int sph_get_kernel_times(sph_handle* h, char* names, int* launches, double* total_ms, int cap);
+int sph_reset_kernel_times(sph_handle* h);
+
+/* -------------------------------------------------------------- samplers */
+/* Each returns the sample count; out (xyzw) receives up to cap samples. */
+
+/* BoxVolume::particles (fluid/simulation/boxvolume.cpp:11-40). */
+int sph_sample_box_volume(const float size[3], const float center[3], float radius, float* out, int cap);
+/* FishTank::surface_sampling (scene/fishtank.cpp:108-141), tank at the origin. */
+int sph_sample_fishtank(float width, float height, float depth, float spacing, float* out, int cap);
+/* Cube::surface_sampling (scene/cube.cpp:136-170), body-local. */
+int sph_sample_cube(float side, float spacing, float* out, int cap);
+/* Wall::surface_sampling (scene/wall.cpp:142-152), body-local. */
+int sph_sample_wall(float width, float height, float spacing, float* out, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPH_H_ */

@@ -1,0 +1,37 @@
+"""Build recipe for the CPU oracle (test infrastructure only).
+
+Compiles ``oracle/sph_oracle.cpp`` into ``oracle/_build/liboracle.so`` with
+g++: IEEE fp32 (no -ffast-math), no FMA contraction, OpenMP. The flags that
+pin the arithmetic are -ffp-contract=off and the absence of -ffast-math;
+-march=x86-64-v3 keeps the library loadable on any AVX2 host.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT_DIR = os.path.join(HERE, "_build")
+LIB = os.path.join(OUT_DIR, "liboracle.so")
+
+FLAGS = [
+    "-std=c++17", "-O3", "-march=x86-64-v3", "-fopenmp",
+    "-ffp-contract=off", "-fno-fast-math", "-fno-math-errno",
+    "-fPIC", "-shared",
+]
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(OUT_DIR, exist_ok=True)
+    src = os.path.join(HERE, "sph_oracle.cpp")
+    tmp = LIB + ".tmp"
+    cmd = ["g++", *FLAGS, src, "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

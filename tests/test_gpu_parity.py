@@ -1,0 +1,139 @@
+"""Parity of the HIP path against the CPU oracle, through the C ABI.
+
+Every case builds the same scene on libsph and on the oracle, puts both on
+the identical input state, runs one step and compares:
+  * integer outputs (cell keys, sort permutation, cell ranges, neighbour
+    counts and lists) bit for bit;
+  * fp32 fields bit for bit as well: both sides evaluate the same IEEE
+    expressions in the same order, so the bar is equality; the 1e-5
+    max-relative tolerance is asserted separately as the contract bound;
+  * the body wrench (a sum whose order differs between the two sides)
+    within 1e-5 of the oracle's double-precision sum, relative to the sum of
+    magnitudes.
+The fp32 error of each field against the oracle's double-precision twin is
+printed (and bounded loosely) so it is on record.
+"""
+import numpy as np
+import pytest
+
+from fluids_amd import scenes
+from helpers import build_oracle, max_rel
+
+pytestmark = pytest.mark.gpu
+
+INT_FIELDS = ["keys", "perm", "ncount", "nlist", "bncount", "bnlist", "bkeys", "bperm"]
+PCI_FIELDS = ["pos_sorted", "vel_sorted", "density", "normal", "force_other", "pressure", "err",
+              "force_pressure", "pos_pred", "pos", "vel"]
+WC_FIELDS = ["pos_sorted", "vel_sorted", "vel_half_sorted", "density", "pressure", "normal", "accel",
+             "pos", "vel", "vel_half"]
+TOL = 1e-5
+
+
+def _same_state(sim, orc, method):
+    """Puts both solvers on the GPU solver's current stored state."""
+    pos = sim.positions()
+    vel = sim.velocities()
+    vh = sim.half_velocities() if method == 0 else None
+    orc.set_state(pos, vel, vh)
+    sim.set_positions(pos)
+    sim.set_velocities(vel)
+    if vh is not None:
+        sim.set_half_velocities(vh)
+    return pos, vel
+
+
+def _compare(sim, orc, fields, label):
+    report = {}
+    for f in INT_FIELDS:
+        a, b = sim.field(f), orc.field(f)
+        assert a.shape == b.shape, (label, f, a.shape, b.shape)
+        assert np.array_equal(a, b), f"{label}: {f} differs at {np.argwhere(a != b)[:5].ravel()}"
+    for f in ["ranges", "branges"]:
+        a, b = sim.field(f), orc.field(f)
+        assert np.array_equal(a, b), f"{label}: {f} differs ({a.shape} vs {b.shape})"
+    for f in fields:
+        a, b = sim.field(f), orc.field(f)
+        if a.ndim == 2:
+            a, b = a[:, :3], b[:, :3]
+        err = max_rel(a, b)
+        report[f] = err
+        assert err <= TOL, f"{label}: {f} max-rel {err:.3e} > {TOL}"
+        nbad = int(np.sum(a.view(np.uint32) != b.view(np.uint32)))
+        report[f + "_bits_differing"] = nbad
+    print(label, report, flush=True)
+    return report
+
+
+def _twin_errors(orc, fields, label):
+    out = {}
+    for f in fields:
+        try:
+            d = orc.field(f, twin=True)
+        except KeyError:
+            continue
+        a = orc.field(f)
+        if a.ndim == 2:
+            a = a[:, :3]
+        out[f] = max_rel(a, d)
+    print(label, "fp32 vs fp64 oracle:", {k: f"{v:.2e}" for k, v in out.items()}, flush=True)
+    return out
+
+
+@pytest.mark.parametrize("cube", [False, True])
+def test_pcisph_single_step_parity(cube):
+    sc = scenes.small_pcisph(cube=cube)
+    print("case pcisph small", sc.n_fluid, sc.n_boundary, "cube", cube, flush=True)
+    sim = scenes.build(sc)
+    orc = build_oracle(sc, twin=True)
+    # the relaxation step at add_volume already ran on both sides
+    assert np.array_equal(sim.positions(), orc.field("pos")), "relaxed initial state differs"
+    for step in range(3):
+        _same_state(sim, orc, 1)
+        if cube:
+            com = np.array([-0.25, -0.25 + 0.001 * step, 0.0], np.float32)
+            R = scenes.quat_to_rows([0, 0.3 + 0.01 * step, 0, 1])
+            sim.set_body_state(1, R=R, com=com, lin=[0, 0.1, 0], ang=[0, 0.2, 0])
+            orc.set_body_state(1, R=R, com=com, lin=[0, 0.1, 0], ang=[0, 0.2, 0])
+        sim.step(1)
+        orc.step(1)
+        assert sim.step_stats()["iterations"] == orc.iterations
+        _compare(sim, orc, PCI_FIELDS, f"pcisph step {step}")
+        if cube:
+            for f in ["bforce", "btorque"]:
+                a, b = sim.field(f)[:, :3], orc.field(f)[:, :3]
+                assert max_rel(a, b) <= TOL, f
+            fg, tg = sim.body_wrench(1)
+            _, _, f64, t64 = orc.body_wrench(1)
+            bf = orc.field("bforce")[:, :3].astype(np.float64)
+            scale = np.abs(bf).sum() + 1e-30
+            assert np.abs(fg - f64).max() <= 1e-5 * scale, (fg, f64)
+    _twin_errors(orc, ["density", "normal", "force_other", "pressure", "force_pressure", "pos", "vel"],
+                 "pcisph")
+
+
+@pytest.mark.parametrize("fix", [True, False])
+def test_wcsph_single_step_parity(fix):
+    sc = scenes.small_wcsph(fix_density=fix)
+    print("case wcsph small", sc.n_fluid, sc.n_boundary, "fix", fix, flush=True)
+    sim = scenes.build(sc)
+    orc = build_oracle(sc, twin=True)
+    for step in range(3):
+        _same_state(sim, orc, 0)
+        sim.step(1)
+        orc.step(1)
+        _compare(sim, orc, WC_FIELDS, f"wcsph step {step} fix={fix}")
+    _twin_errors(orc, ["density", "pressure", "normal", "accel", "pos"], "wcsph")
+
+
+def test_multistep_trajectory_bitexact():
+    """Ten steps from the same initial state stay bit-identical (no reset of
+    the state in between): the sort order chain of the reference (the input
+    order of each step is the previous step's Morton order) is reproduced."""
+    sc = scenes.small_pcisph(seed=9)
+    sim = scenes.build(sc)
+    orc = build_oracle(sc)
+    _same_state(sim, orc, 1)
+    sim.step(10)
+    orc.step(10)
+    assert np.array_equal(sim.positions(), orc.field("pos"))
+    assert np.array_equal(sim.velocities(), orc.field("vel"))
