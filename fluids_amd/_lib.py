@@ -84,6 +84,7 @@ SIGNATURES = {
     "sph_device_state": (I, [P, P, P, P]),
     "sph_stream": (P, [P]),
     "sph_set_profiling": (I, [P, I]),
+    "sph_set_profiling_filter": (I, [P, C.c_char_p]),
     "sph_get_kernel_times": (I, [P, P, P, P, I]),
     "sph_reset_kernel_times": (I, [P]),
     "sph_sample_box_volume": (I, [P, P, F, P, I]),

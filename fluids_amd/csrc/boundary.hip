@@ -74,10 +74,9 @@ int fluid_force_blocks(int count) { return (count + FF_THREADS - 1) / FF_THREADS
 __global__ __launch_bounds__(FF_THREADS) void k_fluid_force(int count, int origin,
                                                             const float4* __restrict__ b_unsorted,
                                                             const float* __restrict__ b_phi, int n,
-                                                            const float4* __restrict__ pos_s,
-                                                            const float* __restrict__ density,
+                                                            const float4* __restrict__ prho,
                                                             const float* __restrict__ pressure,
-                                                            const int4* __restrict__ fR, int fepoch, GridDev g,
+                                                            const int2* __restrict__ fR, GridDev g,
                                                             float H, float spiky_grad, float mass, float cx, float cy,
                                                             float cz, float4* __restrict__ b_force,
                                                             float4* __restrict__ b_torque, float* __restrict__ partial,
@@ -95,12 +94,14 @@ __global__ __launch_bounds__(FF_THREADS) void k_fluid_force(int count, int origi
         for (int o = 0; cnt < LIST_CAP && o < 27; ++o) {
             int dx, dy, dz;
             neigh_offset(o, dx, dy, dz);
-            int2 iv = cell_range(fR, fepoch, cxi + dx, cyi + dy, czi + dz, g, edge);
-            for (int j = iv.x; cnt < LIST_CAP && j < iv.y; ++j) {
-                V3 r = v3(pos_s[j]) - pi;
+            int2 iv = cell_range(fR, cxi + dx, cyi + dy, czi + dz, g, edge);
+            int end = iv.y < n ? iv.y : n;  // ranges never exceed n
+            for (int j = iv.x; cnt < LIST_CAP && j < end; ++j) {
+                float4 q = prho[j];
+                V3 r = v3(q) - pi;
                 float rn = length(r);
                 if (rn < H) {
-                    float C = pressure[j] / sqr(density[j]);
+                    float C = pressure[j] / sqr(q.w);
                     ++cnt;
                     f = f + (((r * (spiky_grad / rn)) * sqr(H - rn)) * 2.0f) * C;
                 }
@@ -126,14 +127,13 @@ __global__ __launch_bounds__(FF_THREADS) void k_fluid_force(int count, int origi
 }
 
 void launch_fluid_force(int count, int origin, const float4* b_unsorted, const float* b_phi, int n,
-                        const float4* pos_s, const float* density, const float* pressure, const int4* fR, int fepoch,
-                        const GridDev& g, float H, float spiky_grad, float mass, const float com[3],
+                        const float4* prho, const float* pressure, const int2* fR, const GridDev& g, float H, float spiky_grad, float mass, const float com[3],
                         float4* b_force, float4* b_torque, float* partial, DevState* st, hipStream_t s,
                         Profiler* prof) {
     if (count <= 0) return;
     PROF_BEGIN(prof, "fluid_force", s);
     hipLaunchKernelGGL(k_fluid_force, dim3(fluid_force_blocks(count)), dim3(FF_THREADS), 0, s, count, origin,
-                       b_unsorted, b_phi, n, pos_s, density, pressure, fR, fepoch, g, H, spiky_grad, mass, com[0],
+                       b_unsorted, b_phi, n, prho, pressure, fR, g, H, spiky_grad, mass, com[0],
                        com[1], com[2], b_force, b_torque, partial, st);
     PROF_END(prof, s);
 }

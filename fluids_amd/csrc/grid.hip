@@ -14,11 +14,20 @@ namespace sph {
 
 // ---------------------------------------------------------------- hashing
 // compute_hashes (grid.cl:20-39). The values array of the sort is the
-// identity; the first sort pass generates it instead of reading it.
+// identity; the first sort pass generates it instead of reading it. Before
+// writing the new key, each thread clears the cell range named by the
+// previous step's sorted key at its index (prev may alias keys).
 __global__ __launch_bounds__(256) void k_hash(int n, const float4* __restrict__ pos, GridDev g,
-                                              uint32_t* __restrict__ keys, DevState* st) {
+                                              uint32_t* keys, const uint32_t* prev, int2* __restrict__ R,
+                                              DevState* st) {
     int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (R) {
+        uint32_t pk = prev[i];
+        int px = (int)compact_by_3(pk), py = (int)compact_by_3(pk >> 1), pz = (int)compact_by_3(pk >> 2);
+        long long b = py < g.cps ? box_index(px, py, pz, g) : -1;
+        if (b >= 0) R[b] = make_int2(0, 0);
+    }
     float4 p = pos[i];
     if (!(isfinite(p.x) && isfinite(p.y) && isfinite(p.z))) atomicAdd(&st->nonfinite, 1);
     int cx, cy, cz;
@@ -29,20 +38,40 @@ __global__ __launch_bounds__(256) void k_hash(int n, const float4* __restrict__ 
 
 // ------------------------------------------------------------- radix sort
 // Stable LSD radix sort of (u32 key, i32 value), 8-bit digits. One pass is
-// histogram -> exclusive scan (digit-major) -> stable scatter. The scatter
-// ranks each 256-key round of a tile with wave64 ballots (the lanes sharing a
-// digit are the AND of 8 bit-ballots), then orders waves through LDS, so the
-// output keeps input order within equal digits.
+//   hist    per-tile digit counts, stored digit-major hist[d][tile]
+//   rowscan one workgroup per digit: exclusive scan of its row, row total
+//   scatter each tile adds the exclusive prefix of the row totals to its
+//           row entries and writes its keys stably
+// The scatter ranks each 256-key round of a tile with wave64 ballots (the
+// lanes sharing a digit are the AND of 8 bit-ballots), then orders waves
+// through LDS, so the output keeps input order within equal digits.
 constexpr int SORT_THREADS = 256;
 constexpr int SORT_ITEMS = 16;
 constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;
-constexpr int RADIX = 256;
+constexpr int RADIX_MAX = 1024;
 
+// Inclusive scan of one value per thread over a 256-thread workgroup.
+__device__ __forceinline__ uint32_t block_incl_scan_256(uint32_t v, uint32_t* wtot) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+    }
+    if (lane == 63) wtot[wave] = v;
+    __syncthreads();
+    uint32_t add = 0;
+    for (int w = 0; w < wave; ++w) add += wtot[w];
+    __syncthreads();
+    return v + add;
+}
+
+template <int BITS>
 __global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint32_t* __restrict__ keys, int n, int shift,
                                                              int ntiles, uint32_t* __restrict__ hist) {
+    constexpr int RADIX = 1 << BITS;
     __shared__ uint32_t h[RADIX];
-    int t = threadIdx.x;
-    h[t] = 0;
+    const int t = threadIdx.x;
+    for (int d = t; d < RADIX; d += SORT_THREADS) h[d] = 0;
     __syncthreads();
     int base = blockIdx.x * SORT_TILE;
 #pragma unroll 4
@@ -51,48 +80,57 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint32_t* __r
         if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & (RADIX - 1)], 1u);
     }
     __syncthreads();
-    hist[(size_t)t * ntiles + blockIdx.x] = h[t];
+    for (int d = t; d < RADIX; d += SORT_THREADS) hist[(size_t)d * ntiles + blockIdx.x] = h[d];
 }
 
-// Exclusive scan of len words in place, one workgroup of 1024 threads: each
-// thread scans a contiguous chunk, chunk totals are scanned in LDS.
-__global__ __launch_bounds__(1024) void k_scan_exclusive(uint32_t* __restrict__ a, int len) {
-    __shared__ uint32_t s[1024];
-    int t = threadIdx.x;
-    int chunk = (len + 1023) / 1024;
-    int b = t * chunk;
-    int e = b + chunk < len ? b + chunk : len;
-    uint32_t sum = 0;
-    for (int i = b; i < e; ++i) sum += a[i];
-    s[t] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint32_t v = (t >= off) ? s[t - off] : 0u;
+__global__ __launch_bounds__(SORT_THREADS) void k_radix_rowscan(uint32_t* __restrict__ hist, int ntiles,
+                                                                uint32_t* __restrict__ totals) {
+    __shared__ uint32_t wtot[4];
+    __shared__ uint32_t chunk_total;
+    uint32_t* row = hist + (size_t)blockIdx.x * ntiles;
+    uint32_t carry = 0;
+    for (int base = 0; base < ntiles; base += SORT_THREADS) {
+        int idx = base + threadIdx.x;
+        uint32_t v = idx < ntiles ? row[idx] : 0u;
+        uint32_t incl = block_incl_scan_256(v, wtot);
+        if (idx < ntiles) row[idx] = carry + incl - v;
+        if (threadIdx.x == SORT_THREADS - 1) chunk_total = incl;
         __syncthreads();
-        s[t] += v;
+        carry += chunk_total;
         __syncthreads();
     }
-    uint32_t run = (t > 0) ? s[t - 1] : 0u;
-    for (int i = b; i < e; ++i) {
-        uint32_t v = a[i];
-        a[i] = run;
-        run += v;
-    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
+template <int BITS>
 __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint32_t* __restrict__ kin,
                                                                 const int* __restrict__ vin,
                                                                 uint32_t* __restrict__ kout, int* __restrict__ vout,
                                                                 int n, int shift, int ntiles,
-                                                                const uint32_t* __restrict__ offs) {
+                                                                const uint32_t* __restrict__ offs,
+                                                                const uint32_t* __restrict__ totals) {
+    constexpr int RADIX = 1 << BITS;
+    constexpr int PER = RADIX / SORT_THREADS;  // digits per thread for the table set-up
+    constexpr int WAVES = SORT_THREADS / 64;
     __shared__ uint32_t base_off[RADIX];
     __shared__ uint32_t run[RADIX];
-    __shared__ uint32_t wcnt[SORT_THREADS / 64][RADIX];
+    __shared__ uint32_t wcnt[WAVES][RADIX];
+    __shared__ uint32_t wtot[4];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    base_off[t] = offs[(size_t)t * ntiles + blockIdx.x];
-    run[t] = 0;
+    // digit bases: exclusive prefix of the row totals (PER consecutive digits per thread)
+    uint32_t tsum = 0;
 #pragma unroll
-    for (int w = 0; w < SORT_THREADS / 64; ++w) wcnt[w][t] = 0;
+    for (int k = 0; k < PER; ++k) tsum += totals[t * PER + k];
+    uint32_t pre = block_incl_scan_256(tsum, wtot) - tsum;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        int d = t * PER + k;
+        base_off[d] = offs[(size_t)d * ntiles + blockIdx.x] + pre;
+        pre += totals[d];
+        run[d] = 0;
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) wcnt[w][d] = 0;
+    }
     __syncthreads();
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     const int base = blockIdx.x * SORT_TILE;
@@ -104,7 +142,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint32_t* 
         uint32_t d = (key >> shift) & (RADIX - 1);
         uint64_t m = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < BITS; ++b) {
             bool bit = (d >> b) & 1u;
             uint64_t bb = __ballot(valid && bit);
             m &= bit ? bb : ~bb;
@@ -116,19 +154,19 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint32_t* 
         __syncthreads();
         uint32_t pos = 0;
         if (valid) {
-            uint32_t pre = run[d];
-            for (int w = 0; w < wave; ++w) pre += wcnt[w][d];
-            pos = base_off[d] + pre + (uint32_t)rank;
+            uint32_t p = run[d];
+            for (int w = 0; w < wave; ++w) p += wcnt[w][d];
+            pos = base_off[d] + p + (uint32_t)rank;
         }
         __syncthreads();
         if (leader) {
             bool highest = true;
-            for (int w = wave + 1; w < SORT_THREADS / 64; ++w)
+            for (int w = wave + 1; w < WAVES; ++w)
                 if (wcnt[w][d]) highest = false;
             if (highest) {
-                uint32_t tot = 0;
-                for (int w = 0; w < SORT_THREADS / 64; ++w) tot += wcnt[w][d];
-                run[d] += tot;
+                uint32_t sum = 0;
+                for (int w = 0; w < WAVES; ++w) sum += wcnt[w][d];
+                run[d] += sum;
             }
         }
         __syncthreads();
@@ -140,18 +178,47 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint32_t* 
     }
 }
 
-int sort_passes(int key_bits) { return key_bits <= 0 ? 1 : (key_bits + 7) / 8; }
+// Digit width per pass: the fewest passes over key_bits, 8..10 bits each.
+static int digit_bits(int key_bits, int& passes) {
+    if (key_bits <= 0) key_bits = 1;
+    passes = (key_bits + 9) / 10;
+    int bits = (key_bits + passes - 1) / passes;
+    return bits < 8 ? 8 : bits;
+}
+
+int sort_passes(int key_bits) {
+    int p;
+    digit_bits(key_bits, p);
+    return p;
+}
 
 size_t sort_hist_words(int n) {
     int ntiles = (n + SORT_TILE - 1) / SORT_TILE;
-    return (size_t)RADIX * (ntiles > 0 ? ntiles : 1);
+    return (size_t)RADIX_MAX * (ntiles > 0 ? ntiles : 1) + RADIX_MAX;
+}
+
+template <int BITS>
+static void radix_pass(const uint32_t* kin, const int* vin, uint32_t* kout, int* vout, uint32_t* hist,
+                       uint32_t* totals, int n, int ntiles, int shift, hipStream_t s, Profiler* prof) {
+    PROF_BEGIN(prof, "radix_hist", s);
+    hipLaunchKernelGGL(k_radix_hist<BITS>, dim3(ntiles), dim3(SORT_THREADS), 0, s, kin, n, shift, ntiles, hist);
+    PROF_END(prof, s);
+    PROF_BEGIN(prof, "radix_rowscan", s);
+    hipLaunchKernelGGL(k_radix_rowscan, dim3(1 << BITS), dim3(SORT_THREADS), 0, s, hist, ntiles, totals);
+    PROF_END(prof, s);
+    PROF_BEGIN(prof, "radix_scatter", s);
+    hipLaunchKernelGGL(k_radix_scatter<BITS>, dim3(ntiles), dim3(SORT_THREADS), 0, s, kin, vin, kout, vout, n, shift,
+                       ntiles, hist, totals);
+    PROF_END(prof, s);
 }
 
 void launch_radix_sort(uint32_t* keys_a, uint32_t* keys_b, int* vals_a, int* vals_b, uint32_t* hist, int n,
                        int key_bits, hipStream_t s, Profiler* prof) {
     if (n <= 0) return;
-    int passes = sort_passes(key_bits);
+    int passes;
+    const int bits = digit_bits(key_bits, passes);
     int ntiles = (n + SORT_TILE - 1) / SORT_TILE;
+    uint32_t* totals = hist + (size_t)RADIX_MAX * ntiles;
     // The caller hashed into keys_b when passes is odd, keys_a when even;
     // either way the result lands in (keys_a, vals_a).
     uint32_t* kin = (passes % 2) ? keys_b : keys_a;
@@ -159,18 +226,10 @@ void launch_radix_sort(uint32_t* keys_a, uint32_t* keys_b, int* vals_a, int* val
     int* vin = nullptr;
     int* vout = (passes % 2) ? vals_a : vals_b;
     for (int p = 0; p < passes; ++p) {
-        int shift = 8 * p;
-        PROF_BEGIN(prof, "radix_hist", s);
-        hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(SORT_THREADS), 0, s, kin, n, shift, ntiles, hist);
-        PROF_END(prof, s);
-        PROF_BEGIN(prof, "radix_scan", s);
-        hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, hist, RADIX * ntiles);
-        PROF_END(prof, s);
-        PROF_BEGIN(prof, "radix_scatter", s);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(SORT_THREADS), 0, s, kin, vin, kout, vout, n, shift,
-                           ntiles, hist);
-        PROF_END(prof, s);
-        // next pass reads what this one wrote
+        int shift = bits * p;
+        if (bits == 8) radix_pass<8>(kin, vin, kout, vout, hist, totals, n, ntiles, shift, s, prof);
+        else if (bits == 9) radix_pass<9>(kin, vin, kout, vout, hist, totals, n, ntiles, shift, s, prof);
+        else radix_pass<10>(kin, vin, kout, vout, hist, totals, n, ntiles, shift, s, prof);
         uint32_t* kt = kin; kin = kout; kout = kt;
         int* nv = (vout == vals_a) ? vals_b : vals_a;
         vin = vout;
@@ -178,11 +237,11 @@ void launch_radix_sort(uint32_t* keys_a, uint32_t* keys_b, int* vals_a, int* val
     }
 }
 
-void launch_hash(int n, const float4* pos, const GridDev& g, uint32_t* keys, DevState* st, hipStream_t s,
-                 Profiler* prof) {
+void launch_hash(int n, const float4* pos, const GridDev& g, uint32_t* keys, const uint32_t* prev, int2* R,
+                 DevState* st, hipStream_t s, Profiler* prof) {
     if (n <= 0) return;
     PROF_BEGIN(prof, "hash", s);
-    hipLaunchKernelGGL(k_hash, dim3((n + 255) / 256), dim3(256), 0, s, n, pos, g, keys, st);
+    hipLaunchKernelGGL(k_hash, dim3((n + 255) / 256), dim3(256), 0, s, n, pos, g, keys, prev, R, st);
     PROF_END(prof, s);
 }
 
@@ -192,7 +251,7 @@ void launch_hash(int n, const float4* pos, const GridDev& g, uint32_t* keys, Dev
 // (grid.cl:60-123): a cell's range starts where the sorted key changes and
 // ends before the next change.
 __device__ __forceinline__ void write_range(int i, int n, const uint32_t* __restrict__ keys_s, float4 p,
-                                            int4* __restrict__ R, const GridDev& g, int epoch) {
+                                            int2* __restrict__ R, const GridDev& g) {
     uint32_t key = keys_s[i];
     bool first = (i == 0) || keys_s[i - 1] != key;
     bool last = (i == n - 1) || keys_s[i + 1] != key;
@@ -201,8 +260,8 @@ __device__ __forceinline__ void write_range(int i, int n, const uint32_t* __rest
         grid_coords(p, g, cx, cy, cz);
         long long b = box_index(cx, cy, cz, g);
         if (b >= 0) {
-            if (first) *reinterpret_cast<int2*>(&R[b]) = make_int2(i, epoch);
-            if (last) R[b].z = i + 1;
+            if (first) R[b].x = i;
+            if (last) R[b].y = i + 1;
         }
     }
 }
@@ -212,7 +271,7 @@ __global__ __launch_bounds__(256) void k_gather_fluid(int n, const int* __restri
                                                       const float4* __restrict__ pos, const float4* __restrict__ vel,
                                                       const float4* __restrict__ vh, float4* __restrict__ pos_s,
                                                       float4* __restrict__ vel_s, float4* __restrict__ vh_s,
-                                                      int4* __restrict__ R, GridDev g, int epoch) {
+                                                      int2* __restrict__ R, GridDev g) {
     int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int j = perm[i];
@@ -221,14 +280,14 @@ __global__ __launch_bounds__(256) void k_gather_fluid(int n, const int* __restri
     pos_s[i] = p;
     vel_s[i] = vel[j];
     if (vh) vh_s[i] = vh[j];
-    write_range(i, n, keys_s, p, R, g, epoch);
+    write_range(i, n, keys_s, p, R, g);
 }
 
 __global__ __launch_bounds__(256) void k_gather_boundary(int n, const int* __restrict__ perm,
                                                          const uint32_t* __restrict__ keys_s,
                                                          const float4* __restrict__ pos, const float* __restrict__ phi,
                                                          float4* __restrict__ pos_s, float* __restrict__ phi_s,
-                                                         int4* __restrict__ R, GridDev g, int epoch) {
+                                                         int2* __restrict__ R, GridDev g) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int j = perm[i];
@@ -236,26 +295,55 @@ __global__ __launch_bounds__(256) void k_gather_boundary(int n, const int* __res
     float4 p = pos[j];
     pos_s[i] = p;
     phi_s[i] = phi[j];
-    write_range(i, n, keys_s, p, R, g, epoch);
+    write_range(i, n, keys_s, p, R, g);
 }
 
 void launch_gather_fluid(int n, const int* perm, const uint32_t* keys_s, const float4* pos, const float4* vel,
-                         const float4* vh, float4* pos_s, float4* vel_s, float4* vh_s, int4* R, const GridDev& g,
-                         int epoch, hipStream_t s, Profiler* prof) {
+                         const float4* vh, float4* pos_s, float4* vel_s, float4* vh_s, int2* R, const GridDev& g,
+                         hipStream_t s, Profiler* prof) {
     if (n <= 0) return;
     PROF_BEGIN(prof, "gather_ranges", s);
     hipLaunchKernelGGL(k_gather_fluid, dim3((n + 255) / 256), dim3(256), 0, s, n, perm, keys_s, pos, vel, vh, pos_s,
-                       vel_s, vh_s, R, g, epoch);
+                       vel_s, vh_s, R, g);
     PROF_END(prof, s);
 }
 
 void launch_gather_boundary(int n, const int* perm, const uint32_t* keys_s, const float4* pos, const float* phi,
-                            float4* pos_s, float* phi_s, int4* R, const GridDev& g, int epoch, hipStream_t s,
-                            Profiler* prof) {
+                            float4* pos_s, float* phi_s, int2* R, const GridDev& g, hipStream_t s, Profiler* prof) {
     if (n <= 0) return;
     PROF_BEGIN(prof, "gather_boundary", s);
     hipLaunchKernelGGL(k_gather_boundary, dim3((n + 255) / 256), dim3(256), 0, s, n, perm, keys_s, pos, phi, pos_s,
-                       phi_s, R, g, epoch);
+                       phi_s, R, g);
+    PROF_END(prof, s);
+}
+
+// Marks, for every interior cell, whether a boundary particle lies in one of
+// its 27 neighbour cells. A fluid particle in an interior cell whose mark is
+// clear has an empty boundary list, so it skips the 27 boundary-range reads.
+__global__ __launch_bounds__(256) void k_mark_boundary(int n, const float4* __restrict__ bpos_s,
+                                                       const uint32_t* __restrict__ keys_s, GridDev g,
+                                                       uint8_t* __restrict__ near) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (i > 0 && keys_s[i - 1] == keys_s[i]) return;  // one thread per occupied cell
+    int cx, cy, cz;
+    grid_coords(bpos_s[i], g, cx, cy, cz);
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dz = -1; dz <= 1; ++dz) {
+                int x = cx - dx, y = cy - dy, z = cz - dz;
+                if (!interior_cell(x, y, z, g.cps)) continue;
+                long long b = box_index(x, y, z, g);
+                if (b >= 0) near[b] = 1;
+            }
+}
+
+void launch_mark_boundary(int n, const float4* bpos_s, const uint32_t* keys_s, const GridDev& g, uint8_t* near,
+                          size_t near_bytes, hipStream_t s, Profiler* prof) {
+    launch_fill_u32(near, near_bytes / 4, 0u, s);
+    if (n <= 0) return;
+    PROF_BEGIN(prof, "mark_boundary", s);
+    hipLaunchKernelGGL(k_mark_boundary, dim3((n + 255) / 256), dim3(256), 0, s, n, bpos_s, keys_s, g, near);
     PROF_END(prof, s);
 }
 
@@ -264,16 +352,78 @@ void launch_gather_boundary(int n, const int* perm, const uint32_t* keys_s, cons
 // order, sorted order inside a cell, strict r2 < CELL_SIZE^2, cap 50,
 // column-major list[k*N + i]. The density kernels read exactly these r2 in
 // exactly this order, so the sum is accumulated while the list is written.
+// The 27 cell ranges are fetched up front and candidates are loaded four at
+// a time, so several loads are in flight per thread; they are consumed in
+// order, which keeps both the list and the sum identical.
+template <bool WCSPH>
+__device__ __forceinline__ void accumulate_density(float r2, float& d, const DensityParams& P, float w,
+                                                   bool weighted) {
+    float t;
+    if (WCSPH) {
+        float rn = P.fix_density ? r2 : sqrtf(r2);
+        t = cube(P.h * P.h - rn);
+    } else {
+        t = cube(P.H2 - r2);
+    }
+    d += weighted ? t * w : t;
+}
+
+constexpr int WALK_BATCH = 8;
+
+template <bool WCSPH, bool PHI>
+__device__ __forceinline__ int walk_cells(V3 pi, int cx, int cy, int cz, const int2* __restrict__ R,
+                                          const float4* __restrict__ P4, const float* __restrict__ phi,
+                                          int count, const GridDev& g, const DensityParams& P,
+                                          int* __restrict__ list, int n, float& d, int& edge) {
+    int len = 0;
+    int2 cur = cell_range(R, cx, cy, cz, g, edge);  // own cell first
+    for (int o = 0; o < 27 && len < LIST_CAP; ++o) {
+        // fetch the next cell's range while this cell is processed
+        int2 nxt = make_int2(0, 0);
+        if (o + 1 < 27) {
+            int dx, dy, dz;
+            neigh_offset(o + 1, dx, dy, dz);
+            nxt = cell_range(R, cx + dx, cy + dy, cz + dz, g, edge);
+        }
+        const int a = cur.x, b = cur.y < count ? cur.y : count;  // ranges never exceed count
+        for (int j0 = a; j0 < b && len < LIST_CAP; j0 += WALK_BATCH) {
+            float4 q[WALK_BATCH];
+            float w[WALK_BATCH];
+#pragma unroll
+            for (int u = 0; u < WALK_BATCH; ++u) {
+                int j = j0 + u < b ? j0 + u : b - 1;
+                q[u] = P4[j];
+                w[u] = PHI ? phi[j] : 1.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < WALK_BATCH; ++u) {
+                int j = j0 + u;
+                if (j < b && len < LIST_CAP) {
+                    V3 r = pi - v3(q[u]);
+                    float r2 = dot(r, r);
+                    if (r2 < g.sqr_support) {
+                        list[(size_t)len * n] = j;
+                        ++len;
+                        accumulate_density<WCSPH>(r2, d, P, w[u], PHI);
+                    }
+                }
+            }
+        }
+        cur = nxt;
+    }
+    return len;
+}
+
 template <bool WCSPH>
 __global__ __launch_bounds__(256) void k_lists_density(int n, const float4* __restrict__ pos_s,
-                                                       const int4* __restrict__ fR, int fepoch, int nb,
+                                                       const int2* __restrict__ fR, int nb,
                                                        const float4* __restrict__ bpos_s,
-                                                       const float* __restrict__ bphi_s, const int4* __restrict__ bR,
-                                                       int bepoch, GridDev g, DensityParams P,
+                                                       const float* __restrict__ bphi_s, const int2* __restrict__ bR,
+                                                       const uint8_t* __restrict__ bnear, GridDev g, DensityParams P,
                                                        int* __restrict__ nl, int* __restrict__ nc,
                                                        int* __restrict__ bnl, int* __restrict__ bnc,
                                                        float* __restrict__ density, float* __restrict__ pressure,
-                                                       DevState* st) {
+                                                       float4* __restrict__ prho, DevState* st) {
     int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float4 p4 = pos_s[i];
@@ -282,49 +432,17 @@ __global__ __launch_bounds__(256) void k_lists_density(int n, const float4* __re
     grid_coords(p4, g, cx, cy, cz);
     int edge = 0;
     float d = 0.0f;
-    int len = 0;
-    for (int o = 0; len < LIST_CAP && o < 27; ++o) {
-        int dx, dy, dz;
-        neigh_offset(o, dx, dy, dz);
-        int2 iv = cell_range(fR, fepoch, cx + dx, cy + dy, cz + dz, g, edge);
-        for (int j = iv.x; len < LIST_CAP && j < iv.y; ++j) {
-            V3 r = pi - v3(pos_s[j]);
-            float r2 = dot(r, r);
-            if (r2 < g.sqr_support) {
-                nl[(size_t)len * n + i] = j;
-                ++len;
-                if (WCSPH) {
-                    float rn = P.fix_density ? r2 : sqrtf(r2);
-                    d += cube(P.h * P.h - rn);
-                } else {
-                    d += cube(P.H2 - r2);
-                }
-            }
-        }
-    }
+    int len = walk_cells<WCSPH, false>(pi, cx, cy, cz, fR, pos_s, nullptr, n, g, P, nl + i, n, d, edge);
     nc[i] = len;
     float bd = 0.0f;
     int blen = 0;
     if (nb > 0) {
-        for (int o = 0; blen < LIST_CAP && o < 27; ++o) {
-            int dx, dy, dz;
-            neigh_offset(o, dx, dy, dz);
-            int2 iv = cell_range(bR, bepoch, cx + dx, cy + dy, cz + dz, g, edge);
-            for (int j = iv.x; blen < LIST_CAP && j < iv.y; ++j) {
-                V3 r = pi - v3(bpos_s[j]);
-                float r2 = dot(r, r);
-                if (r2 < g.sqr_support) {
-                    bnl[(size_t)blen * n + i] = j;
-                    ++blen;
-                    if (WCSPH) {
-                        float rn = P.fix_density ? r2 : sqrtf(r2);
-                        bd += cube(P.h * P.h - rn) * bphi_s[j];
-                    } else {
-                        bd += cube(P.H2 - r2) * bphi_s[j];
-                    }
-                }
-            }
+        bool skip = false;
+        if (interior_cell(cx, cy, cz, g.cps)) {
+            long long b = box_index(cx, cy, cz, g);
+            skip = b >= 0 && bnear[b] == 0;
         }
+        if (!skip) blen = walk_cells<WCSPH, true>(pi, cx, cy, cz, bR, bpos_s, bphi_s, nb, g, P, bnl + i, n, bd, edge);
     }
     bnc[i] = blen;
     if (WCSPH) {
@@ -335,42 +453,55 @@ __global__ __launch_bounds__(256) void k_lists_density(int n, const float4* __re
         pressure[i] = cl_max(0.0f, P.k_gas * (pown7(d / P.rho0) - 1.0f));
     } else {
         // compute_initial_density (pcisph.cl:101)
-        density[i] = P.w_eval * ((d * P.M) + bd);
+        d = P.w_eval * ((d * P.M) + bd);
+        density[i] = d;
     }
+    prho[i] = make_float4(p4.x, p4.y, p4.z, d);
     if (edge) atomicAdd(&st->edge_reads, edge);
 }
 
-void launch_lists_density(bool wcsph, int n, const float4* pos_s, const int4* fR, int fepoch, int nb,
-                          const float4* bpos_s, const float* bphi_s, const int4* bR, int bepoch, const GridDev& g,
+void launch_lists_density(bool wcsph, int n, const float4* pos_s, const int2* fR, int nb, const float4* bpos_s,
+                          const float* bphi_s, const int2* bR, const uint8_t* bnear, const GridDev& g,
                           const DensityParams& P, int* nl, int* nc, int* bnl, int* bnc, float* density,
-                          float* pressure, DevState* st, hipStream_t s, Profiler* prof) {
+                          float* pressure, float4* prho, DevState* st, hipStream_t s, Profiler* prof) {
     if (n <= 0) return;
     PROF_BEGIN(prof, "lists_density", s);
+    const int blocks = (n + 255) / 256;
     if (wcsph)
-        hipLaunchKernelGGL(k_lists_density<true>, dim3((n + 255) / 256), dim3(256), 0, s, n, pos_s, fR, fepoch, nb,
-                           bpos_s, bphi_s, bR, bepoch, g, P, nl, nc, bnl, bnc, density, pressure, st);
+        hipLaunchKernelGGL(k_lists_density<true>, dim3(blocks), dim3(256), 0, s, n, pos_s, fR, nb, bpos_s, bphi_s, bR,
+                           bnear, g, P, nl, nc, bnl, bnc, density, pressure, prho, st);
     else
-        hipLaunchKernelGGL(k_lists_density<false>, dim3((n + 255) / 256), dim3(256), 0, s, n, pos_s, fR, fepoch, nb,
-                           bpos_s, bphi_s, bR, bepoch, g, P, nl, nc, bnl, bnc, density, pressure, st);
+        hipLaunchKernelGGL(k_lists_density<false>, dim3(blocks), dim3(256), 0, s, n, pos_s, fR, nb, bpos_s, bphi_s, bR,
+                           bnear, g, P, nl, nc, bnl, bnc, density, pressure, prho, st);
     PROF_END(prof, s);
 }
 
 // ------------------------------------------------------------------ normals
-// compute_normals (normals.cl:23-70)
-__global__ __launch_bounds__(256) void k_normals(int n, const float4* __restrict__ pos_s,
-                                                 const float* __restrict__ density, const int* __restrict__ nl,
-                                                 const int* __restrict__ nc, float H2, float w_grad, float M, float H,
-                                                 float4* __restrict__ normal) {
+// compute_normals (normals.cl:23-70); prho holds (x, y, z, density).
+__global__ __launch_bounds__(256) void k_normals(int n, const float4* __restrict__ prho,
+                                                 const int* __restrict__ nl, const int* __restrict__ nc, float H2,
+                                                 float w_grad, float M, float H, float4* __restrict__ normal) {
     int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    V3 pi = v3(pos_s[i]);
+    V3 pi = v3(prho[i]);
     V3 acc = {0.0f, 0.0f, 0.0f};
-    int len = nc[i];
-    for (int k = 0; k < len; ++k) {
-        int j = nl[(size_t)k * n + i];
-        V3 r = pi - v3(pos_s[j]);
-        float q = sqr(H2 - dot(r, r));
-        acc = acc + (r * q) / density[j];
+    const int len = nc[i];
+    const int* L = nl + i;
+    for (int k0 = 0; k0 < len; k0 += GATHER_BATCH) {
+        int jj[GATHER_BATCH];
+        float4 q[GATHER_BATCH];
+#pragma unroll
+        for (int u = 0; u < GATHER_BATCH; ++u) jj[u] = k0 + u < len ? L[(size_t)(k0 + u) * n] : i;
+#pragma unroll
+        for (int u = 0; u < GATHER_BATCH; ++u) q[u] = prho[jj[u]];
+#pragma unroll
+        for (int u = 0; u < GATHER_BATCH; ++u) {
+            if (k0 + u < len) {
+                V3 r = pi - v3(q[u]);
+                float qq = sqr(H2 - dot(r, r));
+                acc = acc + (r * qq) / q[u].w;
+            }
+        }
     }
     V3 o = w_grad * acc;
     o = o * M;
@@ -378,12 +509,11 @@ __global__ __launch_bounds__(256) void k_normals(int n, const float4* __restrict
     normal[i] = f4(o, 0.0f);
 }
 
-void launch_normals(int n, const float4* pos_s, const float* density, const int* nl, const int* nc, float H2,
-                    float w_grad, float M, float H, float4* normal, hipStream_t s, Profiler* prof) {
+void launch_normals(int n, const float4* prho, const int* nl, const int* nc, float H2, float w_grad, float M,
+                    float H, float4* normal, hipStream_t s, Profiler* prof) {
     if (n <= 0) return;
     PROF_BEGIN(prof, "normals", s);
-    hipLaunchKernelGGL(k_normals, dim3((n + 255) / 256), dim3(256), 0, s, n, pos_s, density, nl, nc, H2, w_grad, M, H,
-                       normal);
+    hipLaunchKernelGGL(k_normals, dim3((n + 255) / 256), dim3(256), 0, s, n, prho, nl, nc, H2, w_grad, M, H, normal);
     PROF_END(prof, s);
 }
 

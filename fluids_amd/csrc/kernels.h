@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cstddef>
+
 #include <string>
 #include <vector>
 
@@ -17,6 +19,7 @@ struct DevState;
 class Profiler {
 public:
     bool enabled = false;
+    std::string filter;  // when set, only launches of this kernel are timed
     void begin(const char* name, hipStream_t s);
     void end(hipStream_t s);
     void resolve();  // waits for pending events and accumulates their times
@@ -37,6 +40,7 @@ private:
     std::vector<Pending> pending_;
     std::vector<hipEvent_t> pool_;
     int current_ = -1;
+    bool active_ = false;
     hipEvent_t cur_a_ = nullptr;
     hipEvent_t get_event();
 };
@@ -86,46 +90,46 @@ struct BodyDev {
 // grid.hip
 int sort_passes(int key_bits);
 size_t sort_hist_words(int n);
-void launch_hash(int n, const float4* pos, const GridDev& g, uint32_t* keys, DevState* st, hipStream_t s,
-                 Profiler* prof);
+void launch_hash(int n, const float4* pos, const GridDev& g, uint32_t* keys, const uint32_t* prev, int2* R,
+                 DevState* st, hipStream_t s, Profiler* prof);
 void launch_radix_sort(uint32_t* keys_a, uint32_t* keys_b, int* vals_a, int* vals_b, uint32_t* hist, int n,
                        int key_bits, hipStream_t s, Profiler* prof);
 void launch_gather_fluid(int n, const int* perm, const uint32_t* keys_s, const float4* pos, const float4* vel,
-                         const float4* vh, float4* pos_s, float4* vel_s, float4* vh_s, int4* R, const GridDev& g,
-                         int epoch, hipStream_t s, Profiler* prof);
+                         const float4* vh, float4* pos_s, float4* vel_s, float4* vh_s, int2* R, const GridDev& g,
+                         hipStream_t s, Profiler* prof);
 void launch_gather_boundary(int n, const int* perm, const uint32_t* keys_s, const float4* pos, const float* phi,
-                            float4* pos_s, float* phi_s, int4* R, const GridDev& g, int epoch, hipStream_t s,
-                            Profiler* prof);
-void launch_lists_density(bool wcsph, int n, const float4* pos_s, const int4* fR, int fepoch, int nb,
-                          const float4* bpos_s, const float* bphi_s, const int4* bR, int bepoch, const GridDev& g,
+                            float4* pos_s, float* phi_s, int2* R, const GridDev& g, hipStream_t s, Profiler* prof);
+void launch_mark_boundary(int n, const float4* bpos_s, const uint32_t* keys_s, const GridDev& g, uint8_t* near,
+                          size_t near_bytes, hipStream_t s, Profiler* prof);
+void launch_lists_density(bool wcsph, int n, const float4* pos_s, const int2* fR, int nb, const float4* bpos_s,
+                          const float* bphi_s, const int2* bR, const uint8_t* bnear, const GridDev& g,
                           const DensityParams& P, int* nl, int* nc, int* bnl, int* bnc, float* density,
-                          float* pressure, DevState* st, hipStream_t s, Profiler* prof);
-void launch_normals(int n, const float4* pos_s, const float* density, const int* nl, const int* nc, float H2,
-                    float w_grad, float M, float H, float4* normal, hipStream_t s, Profiler* prof);
+                          float* pressure, float4* prho, DevState* st, hipStream_t s, Profiler* prof);
+void launch_normals(int n, const float4* prho, const int* nl, const int* nc, float H2, float w_grad, float M,
+                    float H, float4* normal, hipStream_t s, Profiler* prof);
 void launch_step_begin(DevState* st, hipStream_t s);
 void launch_fill_u32(void* p, size_t n, uint32_t v, hipStream_t s);
 void launch_stats(int n, const int* nc, const int* bnc, const float4* pos, unsigned long long* acc, hipStream_t s);
 
 // pcisph.hip
-void launch_forces_predict(int n, const float4* pos_s, const float4* vel_s, const float* density,
-                           const float4* normal, const int* nl, const int* nc, const PciParams& P, float4* f_other,
-                           float4* pos_pred, hipStream_t s, Profiler* prof);
-void launch_update_pressure(int n, int iter, const float4* pos_pred, const int* nl, const int* nc, const int* bnl,
-                            const int* bnc, const float4* bpos_s, const float* bphi_s, int has_b,
-                            const PciParams& P, float* err, float* pressure, DevState* st, hipStream_t s,
-                            Profiler* prof);
-void launch_pressure_force(int n, int iter, const float4* pos_s, const float4* vel_s, const float* density,
-                           const float* pressure, const float4* f_other, const int* nl, const int* nc,
-                           const int* bnl, const int* bnc, const float4* bpos_s, const float* bphi_s, int has_b,
-                           const PciParams& P, float4* f_press, float4* pos_pred, float4* pos_out, float4* vel_out,
-                           DevState* st, hipStream_t s, Profiler* prof);
+void launch_forces_predict(int n, const float4* prho, const float4* vel_s, const float4* normal, const int* nl,
+                           const int* nc, const PciParams& P, float4* f_other, float4* pos_pred, hipStream_t s,
+                           Profiler* prof);
+void launch_update_pressure(int n, int iter, const float4* pos_pred, const float4* prho, const int* nl,
+                            const int* nc, const int* bnl, const int* bnc, const float4* bpos_s, const float* bphi_s,
+                            int has_b, const PciParams& P, float* err, float* pressure, float4* pc, DevState* st,
+                            hipStream_t s, Profiler* prof);
+void launch_pressure_force(int n, int iter, const float4* pc, const float4* vel_s, const float4* f_other,
+                           const int* nl, const int* nc, const int* bnl, const int* bnc, const float4* bpos_s,
+                           const float* bphi_s, int has_b, const PciParams& P, float4* f_press, float4* pos_pred,
+                           float4* pos_out, float4* vel_out, DevState* st, hipStream_t s, Profiler* prof);
 void launch_pci_integrate_only(int n, const float4* pos_s, const float4* vel_s, const float4* f_other,
                                const PciParams& P, float4* pos_out, float4* vel_out, hipStream_t s,
                                Profiler* prof);
 
 // wcsph.hip
-void launch_accel_integrate(int n, const float4* pos_s, const float4* vel_s, const float4* vh_s,
-                            const float* density, const float* pressure, const float4* normal, const int* nl,
+void launch_accel_integrate(int n, const float4* prho, const float4* vel_s, const float4* vh_s,
+                            const float* pressure, const float4* normal, const int* nl,
                             const int* nc, const int* bnl, const int* bnc, const float4* bpos_s,
                             const float* bphi_s, const WcParams& P, float4* accel, float4* pos_out,
                             float4* vel_out, float4* vh_out, hipStream_t s, Profiler* prof);
@@ -137,7 +141,7 @@ void launch_phi(int n, const float4* p, float H_d, float w_eval, float rho0, flo
                 hipStream_t s, Profiler* prof);
 int fluid_force_blocks(int count);
 void launch_fluid_force(int count, int origin, const float4* b_unsorted, const float* b_phi, int n,
-                        const float4* pos_s, const float* density, const float* pressure, const int4* fR, int fepoch,
+                        const float4* prho, const float* pressure, const int2* fR,
                         const GridDev& g, float H, float spiky_grad, float mass, const float com[3],
                         float4* b_force, float4* b_torque, float* partial, DevState* st, hipStream_t s,
                         Profiler* prof);

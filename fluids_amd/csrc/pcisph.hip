@@ -65,29 +65,46 @@ __device__ __forceinline__ float st_kernel(float r, float h, float c) {
     return 0.0f;
 }
 
-__global__ __launch_bounds__(256) void k_forces_predict(int n, const float4* __restrict__ pos_s,
+constexpr int FORCE_BATCH = 4;
+
+__global__ __launch_bounds__(256) void k_forces_predict(int n, const float4* __restrict__ prho,
                                                         const float4* __restrict__ vel_s,
-                                                        const float* __restrict__ density,
                                                         const float4* __restrict__ normal,
                                                         const int* __restrict__ nl, const int* __restrict__ nc,
                                                         PciParams P, float4* __restrict__ f_other,
                                                         float4* __restrict__ pos_pred) {
     int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    V3 pi = v3(pos_s[i]), vi = v3(vel_s[i]);
-    float di = density[i];
+    float4 pr = prho[i];
+    V3 pi = v3(pr), vi = v3(vel_s[i]);
+    float di = pr.w;
     V3 ni = v3(normal[i]);
     V3 fv = {0.0f, 0.0f, 0.0f}, fcurv = {0.0f, 0.0f, 0.0f}, fcoh = {0.0f, 0.0f, 0.0f};
-    int len = nc[i];
-    for (int k = 0; k < len; ++k) {
-        int j = nl[(size_t)k * n + i];
-        V3 r = pi - v3(pos_s[j]);
-        float l = length(r);
-        float dj = density[j];
-        fv = fv + (v3(vel_s[j]) - vi) * ((P.H - l) / dj);
-        float corr = (2.0f * P.rho0_d) / (di + dj);
-        fcurv = fcurv + (ni - v3(normal[j])) * corr;
-        fcoh = fcoh + (normalize(r) * st_kernel(l, P.H, P.st_term)) * corr;
+    const int len = nc[i];
+    const int* L = nl + i;
+    for (int k0 = 0; k0 < len; k0 += FORCE_BATCH) {
+        int jj[FORCE_BATCH];
+        float4 qp[FORCE_BATCH], qv[FORCE_BATCH], qn[FORCE_BATCH];
+#pragma unroll
+        for (int u = 0; u < FORCE_BATCH; ++u) jj[u] = k0 + u < len ? L[(size_t)(k0 + u) * n] : i;
+#pragma unroll
+        for (int u = 0; u < FORCE_BATCH; ++u) {
+            qp[u] = prho[jj[u]];
+            qv[u] = vel_s[jj[u]];
+            qn[u] = normal[jj[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < FORCE_BATCH; ++u) {
+            if (k0 + u < len) {
+                V3 r = pi - v3(qp[u]);
+                float l = length(r);
+                float dj = qp[u].w;
+                fv = fv + (v3(qv[u]) - vi) * ((P.H - l) / dj);
+                float corr = (2.0f * P.rho0_d) / (di + dj);
+                fcurv = fcurv + (ni - v3(qn[u])) * corr;
+                fcoh = fcoh + (normalize(r) * st_kernel(l, P.H, P.st_term)) * corr;
+            }
+        }
     }
     fv = fv * ((P.k_visc * sqr(P.M)) * (P.visc_lapl / di));
     fcoh = fcoh * (P.M * P.st_main);
@@ -100,24 +117,41 @@ __global__ __launch_bounds__(256) void k_forces_predict(int n, const float4* __r
     pos_pred[i] = f4(pci_predict(pi, vi, pci_accel(zero, fo, P), P), 1.0f);
 }
 
+// update_pressure also packs (x, y, z, p/rho^2) of every particle for the
+// pressure force that follows: p_j/SQR(rho_j) of pcisph.cl:482 and
+// C = p_i/SQR(rho_i) of pcisph.cl:452 are this same value.
 __global__ __launch_bounds__(256) void k_update_pressure(int n, int iter, const float4* __restrict__ pos_pred,
+                                                         const float4* __restrict__ prho,
                                                          const int* __restrict__ nl, const int* __restrict__ nc,
                                                          const int* __restrict__ bnl, const int* __restrict__ bnc,
                                                          const float4* __restrict__ bpos_s,
                                                          const float* __restrict__ bphi_s, int has_b, PciParams P,
                                                          float* __restrict__ err, float* __restrict__ pressure,
-                                                         DevState* st) {
+                                                         float4* __restrict__ pc, DevState* st) {
+    __shared__ uint32_t wmax[4];
     int done = st->done_iter;
     if (done != 0 && done < iter) return;
-    int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    float dv = 0.0f;
-    if (i < n) {
+    const int stride = gridDim.x * blockDim.x;
+    float dvmax = 0.0f;
+    for (int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
         V3 xi = v3(pos_pred[i]);
         float pd = 0.0f, pdb = 0.0f;
-        int len = nc[i];
-        for (int k = 0; k < len; ++k) {
-            V3 r = xi - v3(pos_pred[nl[(size_t)k * n + i]]);
-            pd += cube(P.H2 - dot(r, r));
+        const int len = nc[i];
+        const int* L = nl + i;
+        for (int k0 = 0; k0 < len; k0 += GATHER_BATCH) {
+            int jj[GATHER_BATCH];
+            float4 q[GATHER_BATCH];
+#pragma unroll
+            for (int u = 0; u < GATHER_BATCH; ++u) jj[u] = k0 + u < len ? L[(size_t)(k0 + u) * n] : i;
+#pragma unroll
+            for (int u = 0; u < GATHER_BATCH; ++u) q[u] = pos_pred[jj[u]];
+#pragma unroll
+            for (int u = 0; u < GATHER_BATCH; ++u) {
+                if (k0 + u < len) {
+                    V3 r = xi - v3(q[u]);
+                    pd += cube(P.H2 - dot(r, r));
+                }
+            }
         }
         pd *= P.w_eval * P.M;
         if (has_b) {
@@ -129,27 +163,36 @@ __global__ __launch_bounds__(256) void k_update_pressure(int n, int iter, const 
             }
             pdb *= P.w_eval;
         }
-        dv = cl_max(0.0f, (pd + pdb) - P.rho0_d);
+        float dv = cl_max(0.0f, (pd + pdb) - P.rho0_d);
+        dvmax = dv > dvmax ? dv : dvmax;
         err[i] = dv;
         float p0 = (iter == 1) ? 0.0f : pressure[i];  // zero-filled per step (pcisphsimluation.cpp:118)
-        pressure[i] = p0 + dv * P.delta;
+        float p = p0 + dv * P.delta;
+        pressure[i] = p;
+        float4 pr = prho[i];
+        pc[i] = make_float4(pr.x, pr.y, pr.z, p / sqr(pr.w));
     }
     // dv >= +0, so its bits order like the values
-    uint32_t bits = __float_as_uint(dv);
+    uint32_t bits = __float_as_uint(dvmax);
     for (int off = 32; off > 0; off >>= 1) {
         uint32_t o = __shfl_xor(bits, off);
         bits = o > bits ? o : bits;
     }
-    if ((threadIdx.x & 63) == 0) atomicMax(&st->maxerr[iter & (MAXERR_SLOTS - 1)], bits);
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = bits;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t m = wmax[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = wmax[w] > m ? wmax[w] : m;
+        atomicMax(&st->maxerr[iter & (MAXERR_SLOTS - 1)], m);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_pressure_force(
-    int n, int iter, const float4* __restrict__ pos_s, const float4* __restrict__ vel_s,
-    const float* __restrict__ density, const float* __restrict__ pressure, const float4* __restrict__ f_other,
-    const int* __restrict__ nl, const int* __restrict__ nc, const int* __restrict__ bnl,
-    const int* __restrict__ bnc, const float4* __restrict__ bpos_s, const float* __restrict__ bphi_s, int has_b,
-    PciParams P, float4* __restrict__ f_press, float4* __restrict__ pos_pred, float4* __restrict__ pos_out,
-    float4* __restrict__ vel_out, DevState* st) {
+    int n, int iter, const float4* __restrict__ pc, const float4* __restrict__ vel_s,
+    const float4* __restrict__ f_other, const int* __restrict__ nl, const int* __restrict__ nc,
+    const int* __restrict__ bnl, const int* __restrict__ bnc, const float4* __restrict__ bpos_s,
+    const float* __restrict__ bphi_s, int has_b, PciParams P, float4* __restrict__ f_press,
+    float4* __restrict__ pos_pred, float4* __restrict__ pos_out, float4* __restrict__ vel_out, DevState* st) {
     int done = st->done_iter;
     if (done != 0 && done < iter) return;
     float mx = __uint_as_float(st->maxerr[iter & (MAXERR_SLOTS - 1)]);
@@ -161,20 +204,31 @@ __global__ __launch_bounds__(256) void k_pressure_force(
         }
         st->maxerr[(iter + 1) & (MAXERR_SLOTS - 1)] = 0u;
     }
-    int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    V3 xi = v3(pos_s[i]);
-    float di = density[i];
-    float C = pressure[i] / sqr(di);
+    const int stride = gridDim.x * blockDim.x;
+    for (int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    float4 ci = pc[i];
+    V3 xi = v3(ci);
+    float C = ci.w;
     V3 f = {0.0f, 0.0f, 0.0f}, fb = {0.0f, 0.0f, 0.0f};
-    int len = nc[i];
-    for (int k = 0; k < len; ++k) {
-        int j = nl[(size_t)k * n + i];
-        V3 r = xi - v3(pos_s[j]);
-        float l = length(r);
-        if (l >= 1e-5f) {
-            V3 gw = (r * (1.0f / l)) * sqr(P.H - l);
-            f = f + (C + pressure[j] / sqr(density[j])) * gw;
+    const int len = nc[i];
+    const int* L = nl + i;
+    for (int k0 = 0; k0 < len; k0 += GATHER_BATCH) {
+        int jj[GATHER_BATCH];
+        float4 q[GATHER_BATCH];
+#pragma unroll
+        for (int u = 0; u < GATHER_BATCH; ++u) jj[u] = k0 + u < len ? L[(size_t)(k0 + u) * n] : i;
+#pragma unroll
+        for (int u = 0; u < GATHER_BATCH; ++u) q[u] = pc[jj[u]];
+#pragma unroll
+        for (int u = 0; u < GATHER_BATCH; ++u) {
+            if (k0 + u < len) {
+                V3 r = xi - v3(q[u]);
+                float l = length(r);
+                if (l >= 1e-5f) {
+                    V3 gw = (r * (1.0f / l)) * sqr(P.H - l);
+                    f = f + (C + q[u].w) * gw;
+                }
+            }
         }
     }
     f = f * ((-sqr(P.M)) * P.press_grad);
@@ -202,6 +256,7 @@ __global__ __launch_bounds__(256) void k_pressure_force(
         vel_out[i] = f4(v, 0.0f);
     }
 }
+}
 
 // Final integration with a zero pressure force, for a loop of zero corrections.
 __global__ __launch_bounds__(256) void k_integrate_only(int n, const float4* __restrict__ pos_s,
@@ -217,37 +272,46 @@ __global__ __launch_bounds__(256) void k_integrate_only(int n, const float4* __r
     vel_out[i] = f4(v, 0.0f);
 }
 
-void launch_forces_predict(int n, const float4* pos_s, const float4* vel_s, const float* density,
-                           const float4* normal, const int* nl, const int* nc, const PciParams& P, float4* f_other,
-                           float4* pos_pred, hipStream_t s, Profiler* prof) {
+// Corrections up to min_iter always run and get one thread per particle.
+// Later ones run only when the exit test failed, which the reference's
+// parameters make rare (its test passes unless a density error reaches
+// rho0 * (1 + eta)); they are launched on a small grid that loops over the
+// particles, so a correction that returns at once costs little.
+static int correction_blocks(int n, int iter, const PciParams& P) {
+    int full = (n + 255) / 256;
+    if (iter <= P.min_iter) return full;
+    return full < 512 ? full : 512;
+}
+
+void launch_forces_predict(int n, const float4* prho, const float4* vel_s, const float4* normal, const int* nl,
+                           const int* nc, const PciParams& P, float4* f_other, float4* pos_pred, hipStream_t s,
+                           Profiler* prof) {
     if (n <= 0) return;
     PROF_BEGIN(prof, "forces_predict", s);
-    hipLaunchKernelGGL(k_forces_predict, dim3((n + 255) / 256), dim3(256), 0, s, n, pos_s, vel_s, density, normal, nl,
-                       nc, P, f_other, pos_pred);
+    hipLaunchKernelGGL(k_forces_predict, dim3((n + 255) / 256), dim3(256), 0, s, n, prho, vel_s, normal, nl, nc, P,
+                       f_other, pos_pred);
     PROF_END(prof, s);
 }
 
-void launch_update_pressure(int n, int iter, const float4* pos_pred, const int* nl, const int* nc, const int* bnl,
-                            const int* bnc, const float4* bpos_s, const float* bphi_s, int has_b,
-                            const PciParams& P, float* err, float* pressure, DevState* st, hipStream_t s,
-                            Profiler* prof) {
+void launch_update_pressure(int n, int iter, const float4* pos_pred, const float4* prho, const int* nl,
+                            const int* nc, const int* bnl, const int* bnc, const float4* bpos_s, const float* bphi_s,
+                            int has_b, const PciParams& P, float* err, float* pressure, float4* pc, DevState* st,
+                            hipStream_t s, Profiler* prof) {
     if (n <= 0) return;
-    PROF_BEGIN(prof, "update_pressure", s);
-    hipLaunchKernelGGL(k_update_pressure, dim3((n + 255) / 256), dim3(256), 0, s, n, iter, pos_pred, nl, nc, bnl, bnc,
-                       bpos_s, bphi_s, has_b, P, err, pressure, st);
+    PROF_BEGIN(prof, iter > P.min_iter ? "update_pressure_tail" : "update_pressure", s);
+    hipLaunchKernelGGL(k_update_pressure, dim3(correction_blocks(n, iter, P)), dim3(256), 0, s, n, iter, pos_pred, prho, nl, nc,
+                       bnl, bnc, bpos_s, bphi_s, has_b, P, err, pressure, pc, st);
     PROF_END(prof, s);
 }
 
-void launch_pressure_force(int n, int iter, const float4* pos_s, const float4* vel_s, const float* density,
-                           const float* pressure, const float4* f_other, const int* nl, const int* nc,
-                           const int* bnl, const int* bnc, const float4* bpos_s, const float* bphi_s, int has_b,
-                           const PciParams& P, float4* f_press, float4* pos_pred, float4* pos_out, float4* vel_out,
-                           DevState* st, hipStream_t s, Profiler* prof) {
+void launch_pressure_force(int n, int iter, const float4* pc, const float4* vel_s, const float4* f_other,
+                           const int* nl, const int* nc, const int* bnl, const int* bnc, const float4* bpos_s,
+                           const float* bphi_s, int has_b, const PciParams& P, float4* f_press, float4* pos_pred,
+                           float4* pos_out, float4* vel_out, DevState* st, hipStream_t s, Profiler* prof) {
     if (n <= 0) return;
-    PROF_BEGIN(prof, "pressure_force", s);
-    hipLaunchKernelGGL(k_pressure_force, dim3((n + 255) / 256), dim3(256), 0, s, n, iter, pos_s, vel_s, density,
-                       pressure, f_other, nl, nc, bnl, bnc, bpos_s, bphi_s, has_b, P, f_press, pos_pred, pos_out,
-                       vel_out, st);
+    PROF_BEGIN(prof, iter > P.min_iter ? "pressure_force_tail" : "pressure_force", s);
+    hipLaunchKernelGGL(k_pressure_force, dim3(correction_blocks(n, iter, P)), dim3(256), 0, s, n, iter, pc, vel_s, f_other, nl, nc,
+                       bnl, bnc, bpos_s, bphi_s, has_b, P, f_press, pos_pred, pos_out, vel_out, st);
     PROF_END(prof, s);
 }
 

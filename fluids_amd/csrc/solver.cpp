@@ -34,6 +34,8 @@ hipEvent_t Profiler::get_event() {
     return e;
 }
 void Profiler::begin(const char* name, hipStream_t s) {
+    active_ = filter.empty() || filter == name;
+    if (!active_) return;
     int idx = -1;
     for (size_t k = 0; k < entries.size(); ++k)
         if (entries[k].name == name) idx = (int)k;
@@ -46,6 +48,8 @@ void Profiler::begin(const char* name, hipStream_t s) {
     hip_check(hipEventRecord(cur_a_, s), "hipEventRecord");
 }
 void Profiler::end(hipStream_t s) {
+    if (!active_) return;
+    active_ = false;
     hipEvent_t b = get_event();
     hip_check(hipEventRecord(b, s), "hipEventRecord");
     pending_.push_back({current_, cur_a_, b});
@@ -172,6 +176,7 @@ void Grid::reset(float h) {
     have_box = false;
     x0 = z0 = 0;
     x1 = z1 = -1;
+    ++version;
     update_dev();
 }
 
@@ -204,6 +209,7 @@ bool Grid::include(const std::vector<float4>& pts, int margin) {
     bool changed = !have_box || nx0 != x0 || nx1 != x1 || nz0 != z0 || nz1 != z1;
     x0 = nx0; x1 = nx1; z0 = nz0; z1 = nz1;
     have_box = true;
+    if (changed) ++version;
     update_dev();
     return changed;
 }
@@ -272,10 +278,13 @@ void BoundaryHandler::set_body_state(int id, const float* R, const float* com, c
 
 void BoundaryHandler::realloc_ranges() {
     size_t cells = sim_.grid_.box_cells();
-    if (d_ranges.size() != cells || !d_ranges.get()) {
+    if (d_ranges.size() != cells || !d_ranges.get() || box_version != sim_.grid_.version) {
+        d_ranges.release();
         d_ranges.resize(cells);
-        if (cells) hip_check(hipMemsetAsync(d_ranges.get(), 0, cells * sizeof(int4), sim_.stream_), "hipMemsetAsync");
-        epoch = 0;
+        d_near.resize((cells + 3) / 4 * 4);
+        box_version = sim_.grid_.version;
+        sorts = 0;
+        if (cells) hip_check(hipMemsetAsync(d_ranges.get(), 0, cells * sizeof(int2), sim_.stream_), "hipMemsetAsync");
     }
 }
 
@@ -299,6 +308,8 @@ void BoundaryHandler::resample() {
     d_vals_a.resize(B);
     d_vals_b.resize(B);
     d_hist.resize(sort_hist_words(B));
+    d_ranges.release();
+    sorts = 0;
     if (B == 0) return;
     hip_check(hipMemcpyAsync(d_raw.get(), raw_.data(), B * sizeof(float4), hipMemcpyHostToDevice, s), "upload");
     hip_check(hipMemsetAsync(d_force.get(), 0, B * sizeof(float4), s), "memset");
@@ -344,12 +355,16 @@ void BoundaryHandler::sort_all() {
     realloc_ranges();
     int passes = sort_passes(g.key_bits);
     uint32_t* hash_out = (passes % 2) ? d_keys_b.get() : d_keys_a.get();
-    launch_hash(B, d_unsorted.get(), g.dev, hash_out, sim_.d_state.get(), s, &sim_.prof_);
+    // the previous sort's keys (in d_keys_a) name the cells to clear; none on
+    // the first sort after an allocation, whose box is freshly zeroed
+    launch_hash(B, d_unsorted.get(), g.dev, hash_out, d_keys_a.get(), sorts > 0 ? d_ranges.get() : nullptr,
+                sim_.d_state.get(), s, &sim_.prof_);
     launch_radix_sort(d_keys_a.get(), d_keys_b.get(), d_vals_a.get(), d_vals_b.get(), d_hist.get(), B, g.key_bits,
                       s, &sim_.prof_);
-    ++epoch;
     launch_gather_boundary(B, d_vals_a.get(), d_keys_a.get(), d_unsorted.get(), d_phi.get(), d_pos_s.get(),
-                           d_phi_s.get(), d_ranges.get(), g.dev, epoch, s, &sim_.prof_);
+                           d_phi_s.get(), d_ranges.get(), g.dev, s, &sim_.prof_);
+    launch_mark_boundary(B, d_pos_s.get(), d_keys_a.get(), g.dev, d_near.get(), d_near.size(), s, &sim_.prof_);
+    ++sorts;
 }
 
 // BoundaryHandler::sync (boundaryhandler.cpp:251-294). The reference re-sorts
@@ -366,7 +381,7 @@ void BoundaryHandler::sync(bool all) {
             any = true;
         }
     }
-    if (any || !d_ranges.get() || epoch == 0) sort_all();
+    if (any || !d_ranges.get() || sorts == 0) sort_all();
 }
 
 // apply_fluid_forces (boundaryhandler.cpp:296-349) for bodies with mass > 0.
@@ -390,9 +405,9 @@ void BoundaryHandler::apply_fluid_forces() {
         const Body& b = bodies_[k];
         if (!(b.mass > 0.0f)) continue;
         any = true;
-        launch_fluid_force(b.count, b.origin, d_unsorted.get(), d_phi.get(), sim_.n_, sim_.d_pos_s.get(),
-                           sim_.d_density.get(), sim_.d_pressure.get(), sim_.d_ranges.get(), sim_.fluid_epoch,
-                           sim_.grid_.dev, H, spiky, mass, b.com, d_force.get(), d_torque.get(),
+        launch_fluid_force(b.count, b.origin, d_unsorted.get(), d_phi.get(), sim_.n_, sim_.d_prho.get(),
+                           sim_.d_pressure.get(), sim_.d_ranges.get(), sim_.grid_.dev, H, spiky, mass, b.com,
+                           d_force.get(), d_torque.get(),
                            d_partial.get() + (size_t)offs[k] * 8, sim_.d_state.get(), s, &sim_.prof_);
     }
     if (any) launch_wrench_reduce((int)bodies_.size(), d_offsets.get(), d_nblocks.get(), d_partial.get(),
@@ -478,6 +493,8 @@ void SimulationBase::alloc_fluid(int n) {
     size_t N = (size_t)std::max(n, 1);
     d_pos.resize(N); d_vel.resize(N); d_pos_s.resize(N); d_vel_s.resize(N);
     d_normal.resize(N); d_f_other.resize(N); d_f_press.resize(N); d_pos_pred.resize(N);
+    d_prho.resize(N); d_pc.resize(N);
+    fresh_ranges_ = true;
     d_density.resize(N); d_pressure.resize(N); d_err.resize(N);
     d_keys_a.resize(N); d_keys_b.resize(N); d_vals_a.resize(N); d_vals_b.resize(N);
     d_hist.resize(sort_hist_words(n));
@@ -502,12 +519,15 @@ void SimulationBase::ensure_box(const std::vector<float4>& pts) {
 // (Re)allocates the range boxes after the box grew; called before each step.
 void SimulationBase::ensure_ranges() {
     size_t cells = grid_.box_cells();
-    if (!d_ranges.get() || d_ranges.size() != cells) {
+    if (!d_ranges.get() || d_ranges.size() != cells || box_version_ != grid_.version) {
+        d_ranges.release();
         d_ranges.resize(cells);
-        if (cells) hip_check(hipMemsetAsync(d_ranges.get(), 0, cells * sizeof(int4), stream_), "memset");
-        fluid_epoch = 0;
+        if (cells) hip_check(hipMemsetAsync(d_ranges.get(), 0, cells * sizeof(int2), stream_), "memset");
+        fresh_ranges_ = true;
+        box_version_ = grid_.version;
     }
-    if (boundary_.count() > 0 && (!boundary_.d_ranges.get() || boundary_.d_ranges.size() != cells))
+    if (boundary_.count() > 0 &&
+        (!boundary_.d_ranges.get() || boundary_.box_version != grid_.version))
         boundary_.sync(true);
 }
 
@@ -552,14 +572,15 @@ void SimulationBase::require_ready() const {
 void SimulationBase::neighbour_search() {
     const int passes = sort_passes(grid_.key_bits);
     uint32_t* hash_out = (passes % 2) ? d_keys_b.get() : d_keys_a.get();
-    launch_hash(n_, d_pos.get(), grid_.dev, hash_out, d_state.get(), stream_, &prof_);
+    // the previous step's sorted keys (d_keys_a) name the cells to clear
+    launch_hash(n_, d_pos.get(), grid_.dev, hash_out, d_keys_a.get(), fresh_ranges_ ? nullptr : d_ranges.get(),
+                d_state.get(), stream_, &prof_);
+    fresh_ranges_ = false;
     launch_radix_sort(d_keys_a.get(), d_keys_b.get(), d_vals_a.get(), d_vals_b.get(), d_hist.get(), n_,
                       grid_.key_bits, stream_, &prof_);
-    ++fluid_epoch;
     launch_gather_fluid(n_, d_vals_a.get(), d_keys_a.get(), d_pos.get(), d_vel.get(),
                         s_.method == SPH_WCSPH ? d_vh.get() : nullptr, d_pos_s.get(), d_vel_s.get(),
-                        s_.method == SPH_WCSPH ? d_vh_s.get() : nullptr, d_ranges.get(), grid_.dev, fluid_epoch,
-                        stream_, &prof_);
+                        s_.method == SPH_WCSPH ? d_vh_s.get() : nullptr, d_ranges.get(), grid_.dev, stream_, &prof_);
 }
 
 void SimulationBase::step_dt(float dt) {
@@ -596,7 +617,7 @@ void SimulationBase::set_state(int which, const float* xyzw, int n) {
 
 template <class T> static long long copy_out(const DeviceArray<T>& a, size_t n, void* out, long long cap,
                                              hipStream_t s) {
-    if (!out) return (long long)n;
+    if (!out || n == 0) return (long long)n;
     if ((long long)n > cap) throw Error(SPH_ERR_RANGE, "output buffer too small");
     if (!a.get()) throw Error(SPH_ERR_STATE, "field not available");
     if (n) hip_check(hipMemcpyAsync(out, a.get(), n * sizeof(T), hipMemcpyDeviceToHost, s), "download");
@@ -662,14 +683,14 @@ long long SimulationBase::get_field(int field, void* out, long long cap) {
             const auto& K = fl ? d_keys_a : boundary_.d_keys_a;
             const auto& P = fl ? d_pos_s : boundary_.d_pos_s;
             const auto& R = fl ? d_ranges : boundary_.d_ranges;
-            int ep = fl ? fluid_epoch : boundary_.epoch;
             std::vector<uint32_t> keys(m);
             std::vector<float4> pos(m);
-            std::vector<int4> box(R.size());
+            std::vector<int2> box(R.size());
             if (m) {
                 hip_check(hipMemcpyAsync(keys.data(), K.get(), m * 4, hipMemcpyDeviceToHost, s), "download");
                 hip_check(hipMemcpyAsync(pos.data(), P.get(), m * 16, hipMemcpyDeviceToHost, s), "download");
-                hip_check(hipMemcpyAsync(box.data(), R.get(), R.size() * 16, hipMemcpyDeviceToHost, s), "download");
+                hip_check(hipMemcpyAsync(box.data(), R.get(), R.size() * sizeof(int2), hipMemcpyDeviceToHost, s),
+                          "download");
             }
             hip_check(hipStreamSynchronize(s), "sync");
             std::vector<int> tri;
@@ -679,13 +700,12 @@ long long SimulationBase::get_field(int field, void* out, long long cap) {
                 grid_.cell_of(pos[i], cx, cy, cz);
                 long long id = (long long)grid_.cps * ((long long)grid_.cps * cz + cy) + cx;
                 int lx = cx - grid_.x0, lz = cz - grid_.z0;
-                int4 e = make_int4(0, 0, 0, 0);
+                int2 e = make_int2(0, 0);
                 if (lx >= 0 && lx < grid_.dev.bnx && lz >= 0 && lz < grid_.dev.bnz)
                     e = box[((size_t)lz * grid_.cps + cy) * grid_.dev.bnx + lx];
-                int st = e.y == ep ? e.x : 0, en = e.y == ep ? e.z : 0;
                 tri.push_back((int)id);
-                tri.push_back(st);
-                tri.push_back(en);
+                tri.push_back(e.x);
+                tri.push_back(e.y);
             }
             long long cnt = (long long)tri.size() / 3;
             if (!out) return cnt;
@@ -828,35 +848,35 @@ void PCISPHSimulation::simulate_step(int min_iter, int max_iter) {
     D.H2 = P_.H2;
     D.M = P_.M;
     D.w_eval = P_.w_eval;
-    launch_lists_density(false, n_, d_pos_s.get(), d_ranges.get(), fluid_epoch, nb, boundary_.d_pos_s.get(),
-                         boundary_.d_phi_s.get(), boundary_.d_ranges.get(), boundary_.epoch, grid_.dev, D, d_nl.get(),
-                         d_nc.get(), d_bnl.get(), d_bnc.get(), d_density.get(), nullptr, d_state.get(), s, &prof_);
-    launch_normals(n_, d_pos_s.get(), d_density.get(), d_nl.get(), d_nc.get(), P_.H2, P_.w_grad, P_.M, P_.H,
-                   d_normal.get(), s, &prof_);
+    launch_lists_density(false, n_, d_pos_s.get(), d_ranges.get(), nb, boundary_.d_pos_s.get(),
+                         boundary_.d_phi_s.get(), boundary_.d_ranges.get(), boundary_.d_near.get(), grid_.dev, D,
+                         d_nl.get(), d_nc.get(), d_bnl.get(), d_bnc.get(), d_density.get(), nullptr, d_prho.get(),
+                         d_state.get(), s, &prof_);
+    launch_normals(n_, d_prho.get(), d_nl.get(), d_nc.get(), P_.H2, P_.w_grad, P_.M, P_.H, d_normal.get(), s,
+                   &prof_);
     if (max_iter <= 0) {
         launch_fill_u32(d_pressure.get(), n_, 0u, s);
         launch_fill_u32(d_f_press.get(), (size_t)n_ * 4, 0u, s);
-        launch_forces_predict(n_, d_pos_s.get(), d_vel_s.get(), d_density.get(), d_normal.get(), d_nl.get(),
-                              d_nc.get(), P_, d_f_other.get(), d_pos_pred.get(), s, &prof_);
+        launch_forces_predict(n_, d_prho.get(), d_vel_s.get(), d_normal.get(), d_nl.get(), d_nc.get(), P_,
+                              d_f_other.get(), d_pos_pred.get(), s, &prof_);
         boundary_.apply_fluid_forces();
         launch_pci_integrate_only(n_, d_pos_s.get(), d_vel_s.get(), d_f_other.get(), P_, d_pos.get(), d_vel.get(), s,
                                   &prof_);
         return;
     }
-    launch_forces_predict(n_, d_pos_s.get(), d_vel_s.get(), d_density.get(), d_normal.get(), d_nl.get(), d_nc.get(),
-                          P_, d_f_other.get(), d_pos_pred.get(), s, &prof_);
+    launch_forces_predict(n_, d_prho.get(), d_vel_s.get(), d_normal.get(), d_nl.get(), d_nc.get(), P_,
+                          d_f_other.get(), d_pos_pred.get(), s, &prof_);
     // Corrections are launched in chunks without host round trips; launches
     // past the device-side exit return immediately. Only very long loops
     // (the relaxation step's 10000) check the device flag between chunks.
     const int chunk = 8;
     for (int i = 1; i <= max_iter; ++i) {
-        launch_update_pressure(n_, i, d_pos_pred.get(), d_nl.get(), d_nc.get(), d_bnl.get(), d_bnc.get(),
-                               boundary_.d_pos_s.get(), boundary_.d_phi_s.get(), has_b, P_, d_err.get(),
-                               d_pressure.get(), d_state.get(), s, &prof_);
-        launch_pressure_force(n_, i, d_pos_s.get(), d_vel_s.get(), d_density.get(), d_pressure.get(), d_f_other.get(),
-                              d_nl.get(), d_nc.get(), d_bnl.get(), d_bnc.get(), boundary_.d_pos_s.get(),
-                              boundary_.d_phi_s.get(), has_b, P_, d_f_press.get(), d_pos_pred.get(), d_pos.get(),
-                              d_vel.get(), d_state.get(), s, &prof_);
+        launch_update_pressure(n_, i, d_pos_pred.get(), d_prho.get(), d_nl.get(), d_nc.get(), d_bnl.get(),
+                               d_bnc.get(), boundary_.d_pos_s.get(), boundary_.d_phi_s.get(), has_b, P_, d_err.get(),
+                               d_pressure.get(), d_pc.get(), d_state.get(), s, &prof_);
+        launch_pressure_force(n_, i, d_pc.get(), d_vel_s.get(), d_f_other.get(), d_nl.get(), d_nc.get(), d_bnl.get(),
+                              d_bnc.get(), boundary_.d_pos_s.get(), boundary_.d_phi_s.get(), has_b, P_,
+                              d_f_press.get(), d_pos_pred.get(), d_pos.get(), d_vel.get(), d_state.get(), s, &prof_);
         if (i % chunk == 0 && i < max_iter) {
             int done = 0;
             hip_check(hipMemcpyAsync(&done, &d_state.get()->done_iter, sizeof(int), hipMemcpyDeviceToHost, s),
@@ -901,15 +921,15 @@ void WCSPHSimulation::simulate() {
     D.rho0 = s_.rest_density;
     D.k_gas = s_.gas_stiffness;
     D.fix_density = (s_.flags & SPH_FIX_WCSPH_DENSITY) ? 1 : 0;
-    launch_lists_density(true, n_, d_pos_s.get(), d_ranges.get(), fluid_epoch, nb, boundary_.d_pos_s.get(),
-                         boundary_.d_phi_s.get(), boundary_.d_ranges.get(), boundary_.epoch, grid_.dev, D, d_nl.get(),
-                         d_nc.get(), d_bnl.get(), d_bnc.get(), d_density.get(), d_pressure.get(), d_state.get(), s,
-                         &prof_);
+    launch_lists_density(true, n_, d_pos_s.get(), d_ranges.get(), nb, boundary_.d_pos_s.get(),
+                         boundary_.d_phi_s.get(), boundary_.d_ranges.get(), boundary_.d_near.get(), grid_.dev, D,
+                         d_nl.get(), d_nc.get(), d_bnl.get(), d_bnc.get(), d_density.get(), d_pressure.get(),
+                         d_prho.get(), d_state.get(), s, &prof_);
     // compute_normals with the Q8 repair: PARTICLE_MASS is the WCSPH mass
     // printed like every other -D float constant
     float H = define_literal(h);
-    launch_normals(n_, d_pos_s.get(), d_density.get(), d_nl.get(), d_nc.get(), H * H, poly6_grad_,
-                   define_literal(mass_), H, d_normal.get(), s, &prof_);
+    launch_normals(n_, d_prho.get(), d_nl.get(), d_nc.get(), H * H, poly6_grad_, define_literal(mass_), H,
+                   d_normal.get(), s, &prof_);
     boundary_.apply_fluid_forces();
     WcParams P{};
     P.h = h;
@@ -931,8 +951,8 @@ void WCSPHSimulation::simulate() {
     P.lo_y = -container[1] / 2.0f;
     P.lo_z = -container[2] / 2.0f;
     P.hi_z = container[2] / 2.0f;
-    launch_accel_integrate(n_, d_pos_s.get(), d_vel_s.get(), d_vh_s.get(), d_density.get(), d_pressure.get(),
-                           d_normal.get(), d_nl.get(), d_nc.get(), d_bnl.get(), d_bnc.get(), boundary_.d_pos_s.get(),
+    launch_accel_integrate(n_, d_prho.get(), d_vel_s.get(), d_vh_s.get(), d_pressure.get(), d_normal.get(),
+                           d_nl.get(), d_nc.get(), d_bnl.get(), d_bnc.get(), boundary_.d_pos_s.get(),
                            boundary_.d_phi_s.get(), P, d_accel.get(), d_pos.get(), d_vel.get(), d_vh.get(), s, &prof_);
 }
 

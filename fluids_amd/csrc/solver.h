@@ -104,6 +104,7 @@ public:
     int key_bits = 0;
     bool have_box = false;
     int x0 = 0, x1 = -1, z0 = 0, z1 = -1;  // inclusive cell bounds of the box
+    long long version = 0;                 // bumped whenever the box layout changes
 
     void reset(float h);
     void cell_of(float4 p, int& cx, int& cy, int& cz) const;
@@ -145,8 +146,10 @@ public:
     DeviceArray<float> d_phi, d_phi_s, d_partial, d_wrench;
     DeviceArray<uint32_t> d_keys_a, d_keys_b, d_hist;
     DeviceArray<int> d_vals_a, d_vals_b, d_offsets, d_nblocks;
-    DeviceArray<int4> d_ranges;
-    int epoch = 0;
+    DeviceArray<int2> d_ranges;
+    DeviceArray<uint8_t> d_near;  // per box cell: a boundary particle lies in its 27-neighbourhood
+    int sorts = 0;
+    long long box_version = -1;   // grid box layout d_ranges was built for
     std::vector<Body> bodies_;
 
 private:
@@ -207,13 +210,14 @@ public:
     // device fluid buffers (public for the boundary handler)
     DeviceArray<float4> d_pos, d_vel, d_vh, d_pos_s, d_vel_s, d_vh_s;
     DeviceArray<float4> d_normal, d_f_other, d_f_press, d_pos_pred, d_accel;
+    DeviceArray<float4> d_prho;  // (x, y, z, density) of the sorted particles
+    DeviceArray<float4> d_pc;    // (x, y, z, pressure / density^2), PCISPH
     DeviceArray<float> d_density, d_pressure, d_err;
     DeviceArray<uint32_t> d_keys_a, d_keys_b, d_hist;
     DeviceArray<int> d_vals_a, d_vals_b, d_nl, d_nc, d_bnl, d_bnc;
-    DeviceArray<int4> d_ranges;
+    DeviceArray<int2> d_ranges;
     DeviceArray<DevState> d_state;
     DeviceArray<unsigned long long> d_acc;
-    int fluid_epoch = 0;
     float container[3] = {0, 0, 0};
     bool have_container = false;
 
@@ -232,6 +236,8 @@ protected:
     BoundaryHandler boundary_;
     std::vector<std::shared_ptr<FluidVolume>> volumes_;
     int n_ = 0;
+    bool fresh_ranges_ = true;  // the fluid range box holds no ranges yet
+    long long box_version_ = -1;  // grid box layout d_ranges was built for
     hipStream_t stream_ = nullptr;
     Profiler prof_;
     friend class BoundaryHandler;

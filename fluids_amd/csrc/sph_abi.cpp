@@ -305,6 +305,13 @@ int sph_set_profiling(sph_handle* h, int on) {
     });
 }
 
+int sph_set_profiling_filter(sph_handle* h, const char* kernel) {
+    return guard([&] {
+        sim_of(h).profiler().filter = kernel ? kernel : "";
+        return SPH_OK;
+    });
+}
+
 int sph_get_kernel_times(sph_handle* h, char* names, int* launches, double* total_ms, int cap) {
     return guard([&] {
         auto& p = sim_of(h).profiler();

@@ -23,13 +23,15 @@ namespace sph {
 
 constexpr int LIST_CAP = 50;      // NEIGH_LIST_MAX_LENGTH (kernels/common.h:4)
 constexpr int MAXERR_SLOTS = 16;  // ring of per-correction max density errors
+constexpr int GATHER_BATCH = 8;  // neighbour loads issued together per thread
 
 // Uniform grid restated from kernels/grid.h and fluid/simulation/grid.cpp:61-68.
 // Cell ranges live in a dense box: all cps cells in y (the 5 m grid is
 // periodic and the integrators allow y up to 10 m), [bx0, bx0+bnx) in x and
-// [bz0, bz0+bnz) in z. Entries are int4 {start, epoch, end, 0}; an entry whose
-// epoch differs from the current build reads as the empty range [0, 0), which
-// is what the reference's zero-filled int2[cps^3] holds for an empty cell.
+// [bz0, bz0+bnz) in z. Entries are int2 {start, end}; {0, 0} is an empty cell,
+// as in the reference's zero-filled int2[cps^3]. Instead of zero-filling the
+// whole box every step, the hash kernel clears the cells named by the previous
+// step's sorted keys before new ranges are written.
 struct GridDev {
     float half;         // size / 2 = 2.5f
     float cell;         // cell size: full fp32 support radius
@@ -123,8 +125,8 @@ __device__ __forceinline__ long long box_index(int x, int y, int z, const GridDe
 // grid.cl:165-166). CELL_ID is linear, so such a coordinate names another
 // in-range cell unless the linear id leaves [0, cps^3), where the reference
 // reads out of bounds; that case reads as empty and is counted.
-__device__ __forceinline__ int2 cell_range(const int4* __restrict__ R, int epoch, int x, int y, int z,
-                                           const GridDev& g, int& edge) {
+__device__ __forceinline__ int2 cell_range(const int2* __restrict__ R, int x, int y, int z, const GridDev& g,
+                                           int& edge) {
     int c = g.cps;
     if ((unsigned)x >= (unsigned)c || (unsigned)y >= (unsigned)c || (unsigned)z >= (unsigned)c) {
         long long L = (long long)c * ((long long)c * z + y) + x;
@@ -139,8 +141,23 @@ __device__ __forceinline__ int2 cell_range(const int4* __restrict__ R, int epoch
     }
     long long b = box_index(x, y, z, g);
     if (b < 0) return make_int2(0, 0);
-    int4 e = R[b];
-    return (e.y == epoch) ? make_int2(e.x, e.z) : make_int2(0, 0);
+    return R[b];
+}
+
+// Inverse of split_by_3 for 10-bit coordinates (the Morton key is the
+// textbook interleave there).
+__device__ __forceinline__ uint32_t compact_by_3(uint32_t x) {
+    x &= 0x09249249u;
+    x = (x ^ (x >> 2)) & 0x030c30c3u;
+    x = (x ^ (x >> 4)) & 0x0300f00fu;
+    x = (x ^ (x >> 8)) & 0xff0000ffu;
+    x = (x ^ (x >> 16)) & 0x000003ffu;
+    return x;
+}
+
+// Interior cells have all 27 neighbours in [0, cps)^3 without wrapping.
+__device__ __forceinline__ bool interior_cell(int x, int y, int z, int cps) {
+    return x >= 1 && y >= 1 && z >= 1 && x <= cps - 2 && y <= cps - 2 && z <= cps - 2;
 }
 
 // CELL_NEIGH_OFFSET (kernels/grid.h:15-42): own cell, then the 26 others in

@@ -22,18 +22,19 @@ __device__ __forceinline__ float st_kernel_wc(float r, float h, float c) {
     return 0.0f;
 }
 
+constexpr int WC_BATCH = 4;
+
 __global__ __launch_bounds__(256) void k_accel_integrate(
-    int n, const float4* __restrict__ pos_s, const float4* __restrict__ vel_s, const float4* __restrict__ vh_s,
-    const float* __restrict__ density, const float* __restrict__ pressure, const float4* __restrict__ normal,
-    const int* __restrict__ nl, const int* __restrict__ nc, const int* __restrict__ bnl,
-    const int* __restrict__ bnc, const float4* __restrict__ bpos_s, const float* __restrict__ bphi_s, WcParams P,
-    float4* __restrict__ accel, float4* __restrict__ pos_out, float4* __restrict__ vel_out,
-    float4* __restrict__ vh_out) {
+    int n, const float4* __restrict__ prho, const float4* __restrict__ vel_s, const float4* __restrict__ vh_s,
+    const float* __restrict__ pressure, const float4* __restrict__ normal, const int* __restrict__ nl,
+    const int* __restrict__ nc, const int* __restrict__ bnl, const int* __restrict__ bnc,
+    const float4* __restrict__ bpos_s, const float* __restrict__ bphi_s, WcParams P, float4* __restrict__ accel,
+    float4* __restrict__ pos_out, float4* __restrict__ vel_out, float4* __restrict__ vh_out) {
     int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float4 p4 = pos_s[i];
+    float4 p4 = prho[i];
     V3 pi = v3(p4), vi = v3(vel_s[i]);
-    float di = density[i], pri = pressure[i];
+    float di = p4.w, pri = pressure[i];
     V3 ni = v3(normal[i]);
     float C = pri / sqr(di);
     const float h = P.h;
@@ -51,19 +52,36 @@ __global__ __launch_bounds__(256) void k_accel_integrate(
         bacc = bacc + t;
     }
     V3 pacc = {0.0f, 0.0f, 0.0f}, vacc = {0.0f, 0.0f, 0.0f}, fcurv = {0.0f, 0.0f, 0.0f}, fcoh = {0.0f, 0.0f, 0.0f};
-    int len = nc[i];
-    for (int k = 0; k < len; ++k) {
-        int j = nl[(size_t)k * n + i];
-        float dj = density[j], prj = pressure[j];
-        V3 r = pi - v3(pos_s[j]);
-        float rn = length(r);
-        // wcsph.cl:152 compares against the double literal 0.00001
-        if (di > 0.0f && dj > 0.0f && (double)rn > 0.00001) {
-            pacc = pacc + ((r * (P.spiky_grad / rn)) * sqr(h - rn)) * (C + prj / sqr(dj));
-            vacc = vacc + ((v3(vel_s[j]) - vi) * (h - rn)) / dj;
-            float corr = (2.0f * P.rho0) / (di + dj);
-            fcurv = fcurv + (ni - v3(normal[j])) * corr;
-            fcoh = fcoh + (normalize(r) * st_kernel_wc(rn, h, P.st_term)) * corr;
+    const int len = nc[i];
+    const int* L = nl + i;
+    for (int k0 = 0; k0 < len; k0 += WC_BATCH) {
+        int jj[WC_BATCH];
+        float4 qp[WC_BATCH], qv[WC_BATCH], qn[WC_BATCH];
+        float qpr[WC_BATCH];
+#pragma unroll
+        for (int u = 0; u < WC_BATCH; ++u) jj[u] = k0 + u < len ? L[(size_t)(k0 + u) * n] : i;
+#pragma unroll
+        for (int u = 0; u < WC_BATCH; ++u) {
+            qp[u] = prho[jj[u]];
+            qv[u] = vel_s[jj[u]];
+            qn[u] = normal[jj[u]];
+            qpr[u] = pressure[jj[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < WC_BATCH; ++u) {
+            if (k0 + u < len) {
+                float dj = qp[u].w, prj = qpr[u];
+                V3 r = pi - v3(qp[u]);
+                float rn = length(r);
+                // wcsph.cl:152 compares against the double literal 0.00001
+                if (di > 0.0f && dj > 0.0f && (double)rn > 0.00001) {
+                    pacc = pacc + ((r * (P.spiky_grad / rn)) * sqr(h - rn)) * (C + prj / sqr(dj));
+                    vacc = vacc + ((v3(qv[u]) - vi) * (h - rn)) / dj;
+                    float corr = (2.0f * P.rho0) / (di + dj);
+                    fcurv = fcurv + (ni - v3(qn[u])) * corr;
+                    fcoh = fcoh + (normalize(r) * st_kernel_wc(rn, h, P.st_term)) * corr;
+                }
+            }
         }
     }
     pacc = pacc * (-P.M);
@@ -92,20 +110,20 @@ __global__ __launch_bounds__(256) void k_accel_integrate(
         x = cp;
     }
     V3 vt = vh + (P.dt * a) * 0.5f;
-    pos_out[i] = f4(x, p4.w);
+    pos_out[i] = f4(x, 1.0f);  // pos.w stays 1 (wcsph.cl:202 adds dt * 0 to it)
     vh_out[i] = f4(clamp3(vh, -P.max_vel, P.max_vel), 0.0f);
     vel_out[i] = f4(clamp3(vt, -P.max_vel, P.max_vel), 0.0f);
 }
 
-void launch_accel_integrate(int n, const float4* pos_s, const float4* vel_s, const float4* vh_s,
-                            const float* density, const float* pressure, const float4* normal, const int* nl,
-                            const int* nc, const int* bnl, const int* bnc, const float4* bpos_s,
-                            const float* bphi_s, const WcParams& P, float4* accel, float4* pos_out,
-                            float4* vel_out, float4* vh_out, hipStream_t s, Profiler* prof) {
+void launch_accel_integrate(int n, const float4* prho, const float4* vel_s, const float4* vh_s,
+                            const float* pressure, const float4* normal, const int* nl, const int* nc,
+                            const int* bnl, const int* bnc, const float4* bpos_s, const float* bphi_s,
+                            const WcParams& P, float4* accel, float4* pos_out, float4* vel_out, float4* vh_out,
+                            hipStream_t s, Profiler* prof) {
     if (n <= 0) return;
     PROF_BEGIN(prof, "accel_integrate", s);
-    hipLaunchKernelGGL(k_accel_integrate, dim3((n + 255) / 256), dim3(256), 0, s, n, pos_s, vel_s, vh_s, density,
-                       pressure, normal, nl, nc, bnl, bnc, bpos_s, bphi_s, P, accel, pos_out, vel_out, vh_out);
+    hipLaunchKernelGGL(k_accel_integrate, dim3((n + 255) / 256), dim3(256), 0, s, n, prho, vel_s, vh_s, pressure,
+                       normal, nl, nc, bnl, bnc, bpos_s, bphi_s, P, accel, pos_out, vel_out, vh_out);
     PROF_END(prof, s);
 }
 

@@ -213,7 +213,9 @@ class FluidSimulation:
         check(lib().sph_get_step_stats(self._h, C.byref(s)))
         return {k: getattr(s, k) for k, _ in SphStepStats._fields_}
 
-    def set_profiling(self, on=True):
+    def set_profiling(self, on=True, kernel: str = None):
+        """HIP-event timing of every launch, or of one kernel's launches only."""
+        check(lib().sph_set_profiling_filter(self._h, (kernel or "").encode()))
         check(lib().sph_set_profiling(self._h, int(on)))
 
     def kernel_times(self) -> dict:

@@ -233,6 +233,8 @@ void* sph_stream(sph_handle* h);
 
 /* Per-kernel HIP-event timing on the handle's stream (off by default). */
 int sph_set_profiling(sph_handle* h, int on);
+/* Restricts the timing to launches of one kernel (NULL or "" times all). */
+int sph_set_profiling_filter(sph_handle* h, const char* kernel);
 /* Fills names (a buffer of `cap` * 48 chars), launches and total milliseconds
  * per kernel name; returns the number of kernel names. */
 int sph_get_kernel_times(sph_handle* h, char* names, int* launches, double* total_ms, int cap);

@@ -137,3 +137,54 @@ def test_multistep_trajectory_bitexact():
     orc.step(10)
     assert np.array_equal(sim.positions(), orc.field("pos"))
     assert np.array_equal(sim.velocities(), orc.field("vel"))
+
+
+@pytest.mark.parametrize("max_iter,eta", [(7, -1.0), (2, 0.01), (0, 0.01), (12, -1.0)])
+def test_pcisph_iteration_counts(max_iter, eta):
+    """The correction loop of pcisphsimluation.cpp:129-148: a negative error
+    ratio never passes the exit test, so the loop runs to max_iter (past the
+    three corrections that always run); max_iter below three stops there, and
+    zero runs no correction at all."""
+    sc = scenes.small_pcisph(seed=5, cube=True)
+    sc.settings.update(pcisph_max_iterations=max_iter, pcisph_error_ratio=eta)
+    print("case iterations", max_iter, eta, flush=True)
+    sim = scenes.build(sc)
+    orc = build_oracle(sc)
+    for step in range(2):
+        _same_state(sim, orc, 1)
+        sim.step(1)
+        orc.step(1)
+        assert sim.step_stats()["iterations"] == orc.iterations == max(max_iter, 0) or max_iter == 0
+        fields = ["density", "normal", "force_other", "pos", "vel"]
+        if max_iter > 0:
+            fields += ["pressure", "err", "force_pressure", "pos_pred"]
+        _compare(sim, orc, fields, f"iterations {max_iter} step {step}")
+
+
+@pytest.mark.parametrize("method", [1, 0])
+def test_sparse_wrapped_grid(method):
+    """Scattered particles (workgroups whose cell boxes are too large to stage
+    in LDS) spanning the periodic seam of the 5 m grid at y = -2.5 / +2.5:
+    cells wrap (grid.h:74-91), neighbour offsets leave [0, cps) and name other
+    cells through the linear CELL_ID, and ids past cps^3 are counted as edge
+    reads. Everything must still match the oracle bit for bit."""
+    from fluids_amd.scenes import Rng, Scene
+    rng = Rng(77)
+    n = 3000
+    u = rng.uniform(3 * n).reshape(n, 3)
+    lo = np.array([-0.9, -2.95, -0.45])
+    hi = np.array([0.9, -2.05, 0.45])
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = (lo + u * (hi - lo)).astype(np.float32)
+    pos[:, 3] = 1
+    sc = scenes.small_pcisph() if method == 1 else scenes.small_wcsph()
+    sc = Scene("sparse", dict(sc.settings), (2.0, 6.0, 1.0), [pos], [])
+    print("case sparse wrapped", method, flush=True)
+    sim = scenes.build(sc)
+    orc = build_oracle(sc)
+    for step in range(2):
+        _same_state(sim, orc, method)
+        sim.step(1)
+        orc.step(1)
+        _compare(sim, orc, ["density", "pressure", "normal", "pos", "vel"], f"sparse step {step}")
+    assert sim.step_stats()["edge_reads"] >= 0
