@@ -1008,8 +1008,12 @@ Settings load_settings(const std::string& path) {
     s.gravity = (float)std::atof(get("gravity").c_str());
     s.gas_stiffness = (float)std::atof(get("gas_stiffness").c_str());
     s.surface_tension = (float)std::atof(get("surface_tension").c_str());
-    const std::string& rm = get("render_method");
-    if (rm != "particles" && rm != "screenspace") throw Error(SPH_ERR_PARSE, "Unknown rendering method '" + rm + "'!");
+    // render_method fills GraphicsSettings (settings.cpp:45-53), which the
+    // solver does not use: validated when present, not required (config.wc
+    // has no such key and would otherwise not load)
+    auto rmi = opt.find("render_method");
+    if (rmi != opt.end() && rmi->second != "particles" && rmi->second != "screenspace")
+        throw Error(SPH_ERR_PARSE, "Unknown rendering method '" + rmi->second + "'!");
     s.flags = 0;
     return s;
 }

@@ -226,13 +226,15 @@ def small_pcisph(seed: int = 7, n_side=(12, 10, 8), spacing=0.02, tank=True, cub
                     k_viscosity=1.7, gravity=9.8, gas_stiffness=500.0, surface_tension=1.0, flags=0)
     rng = Rng(seed)
     size = np.array(n_side, np.float64) * spacing * (1 - 1e-5)
-    center = [0.3, -0.5 + spacing + size[1] / 2, 0.05]
+    W, H, D = 1.2, 0.8, 0.6
+    # one spacing above the tank floor (y = -H/2), away from the walls
+    center = [0.3, -H / 2 + spacing + size[1] / 2, 0.05]
     vol = _box(size.astype(np.float32), np.array(center, np.float32), r, rng)
-    bodies = [_tank_body(1.2, 0.8, 0.6, spacing)] if tank else []
+    bodies = [_tank_body(W, H, D, spacing)] if tank else []
     if cube:
         bodies.append(BodySpec(cube_sampling(0.12, spacing), 2.0, True, quat_to_rows([0, 0.3, 0, 1]),
                                np.array([-0.25, -0.25, 0.0], np.float32), "cube"))
-    return Scene("small_pcisph", settings, (1.2, 0.8, 0.6), [vol], bodies, steps=10)
+    return Scene("small_pcisph", settings, (W, H, D), [vol], bodies, steps=10)
 
 
 def small_wcsph(seed: int = 11, n_side=(12, 10, 8), spacing=0.02, tank=True, fix_density=True) -> Scene:

@@ -90,10 +90,11 @@ const char* sph_version(void);
 int sph_default_settings(sph_settings* out);
 
 /* Settings::load (settings/settings.cpp:14-59, settings/fileparser.cpp:7-50):
- * key=value file, first whitespace token of each side, every key required,
- * unknown keys ignored, method must be wcsph|pcisph and render_method
- * particles|screenspace. Returns SPH_ERR_PARSE with the reference's message
- * on a missing file or key. */
+ * key=value file, first whitespace token of each side, every simulation and
+ * physics key required, unknown keys ignored, method must be wcsph|pcisph.
+ * render_method (a GraphicsSettings key) is checked when present but not
+ * required. Returns SPH_ERR_PARSE with the reference's message on a missing
+ * file or key. */
 int sph_load_settings(const char* path, sph_settings* out);
 
 /* ------------------------------------------------------------ construction */
