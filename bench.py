@@ -27,6 +27,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 
+from fluids_amd.profiling import read_traffic  # noqa: E402
+
 
 def kernel_bytes(name, K, Kb, n_it):
     """Algorithmic bytes per particle of one launch of each kernel (the
@@ -74,10 +76,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--max-iter", type=int, default=None, help="override pcisph_max_iterations")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-events", action="store_true", help="time without per-kernel events")
     ap.add_argument("--json-out", default=None)
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "pmc_pcisph_throughput_1000k.md"),
+                    help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary (python -m fluids_amd.profiling pmc) "
+                         "that gives the dominant kernel's HBM bytes per launch")
     a = ap.parse_args()
 
     if a.gpus != 1:
@@ -145,7 +150,8 @@ def main():
         alg = kernel_bytes(dom, K, Kb, n_it) * N
         achieved = alg / avg_s
         roof = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK, "traffic": None, "alg_bytes_per_launch": alg,
+                "frac": achieved / HBM_PEAK, "traffic": read_traffic(a.traffic_file, dom),
+                "traffic_source": a.traffic_file, "alg_bytes_per_launch": alg,
                 "avg_launch_us": 1e6 * avg_s, "timed_launches": launches}
 
     cpu = None

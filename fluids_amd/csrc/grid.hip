@@ -369,12 +369,35 @@ __device__ __forceinline__ void accumulate_density(float r2, float& d, const Den
 }
 
 constexpr int WALK_BATCH = 8;
+constexpr int LISTS_THREADS = 64;
+// List slots kept in LDS until the walk ends; slots past it are stored
+// straight to the list. Lanes of a wave find neighbours at different times,
+// so storing each entry when it is found writes a different list row per lane
+// and every row reaches HBM as many partial lines. From LDS the rows go out
+// one at a time, 64 consecutive entries per wave.
+constexpr int STAGE_SLOTS = 24;
+
+struct ListSink {
+    int* stage;  // this thread's column of the LDS stage, stride LISTS_THREADS
+    int* out;    // list + i, stride n
+    int n;
+    __device__ __forceinline__ void put(int k, int j) const {
+        if (k < STAGE_SLOTS)
+            stage[k * LISTS_THREADS] = j;
+        else
+            out[(size_t)k * n] = j;
+    }
+    __device__ __forceinline__ void flush(int len) const {
+        const int m = len < STAGE_SLOTS ? len : STAGE_SLOTS;
+        for (int k = 0; k < m; ++k) out[(size_t)k * n] = stage[k * LISTS_THREADS];
+    }
+};
 
 template <bool WCSPH, bool PHI>
 __device__ __forceinline__ int walk_cells(V3 pi, int cx, int cy, int cz, const int2* __restrict__ R,
                                           const float4* __restrict__ P4, const float* __restrict__ phi,
                                           int count, const GridDev& g, const DensityParams& P,
-                                          int* __restrict__ list, int n, float& d, int& edge) {
+                                          const ListSink& list, float& d, int& edge) {
     int len = 0;
     int2 cur = cell_range(R, cx, cy, cz, g, edge);  // own cell first
     for (int o = 0; o < 27 && len < LIST_CAP; ++o) {
@@ -402,7 +425,7 @@ __device__ __forceinline__ int walk_cells(V3 pi, int cx, int cy, int cz, const i
                     V3 r = pi - v3(q[u]);
                     float r2 = dot(r, r);
                     if (r2 < g.sqr_support) {
-                        list[(size_t)len * n] = j;
+                        list.put(len, j);
                         ++len;
                         accumulate_density<WCSPH>(r2, d, P, w[u], PHI);
                     }
@@ -415,15 +438,12 @@ __device__ __forceinline__ int walk_cells(V3 pi, int cx, int cy, int cz, const i
 }
 
 template <bool WCSPH>
-__global__ __launch_bounds__(256) void k_lists_density(int n, const float4* __restrict__ pos_s,
-                                                       const int2* __restrict__ fR, int nb,
-                                                       const float4* __restrict__ bpos_s,
-                                                       const float* __restrict__ bphi_s, const int2* __restrict__ bR,
-                                                       const uint8_t* __restrict__ bnear, GridDev g, DensityParams P,
-                                                       int* __restrict__ nl, int* __restrict__ nc,
-                                                       int* __restrict__ bnl, int* __restrict__ bnc,
-                                                       float* __restrict__ density, float* __restrict__ pressure,
-                                                       float4* __restrict__ prho, DevState* st) {
+__global__ __launch_bounds__(LISTS_THREADS) void k_lists_density(
+    int n, const float4* __restrict__ pos_s, const int2* __restrict__ fR, int nb, const float4* __restrict__ bpos_s,
+    const float* __restrict__ bphi_s, const int2* __restrict__ bR, const uint8_t* __restrict__ bnear, GridDev g,
+    DensityParams P, int* __restrict__ nl, int* __restrict__ nc, int* __restrict__ bnl, int* __restrict__ bnc,
+    float* __restrict__ density, float* __restrict__ pressure, float4* __restrict__ prho, DevState* st) {
+    __shared__ int stage[STAGE_SLOTS * LISTS_THREADS];
     int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float4 p4 = pos_s[i];
@@ -432,7 +452,10 @@ __global__ __launch_bounds__(256) void k_lists_density(int n, const float4* __re
     grid_coords(p4, g, cx, cy, cz);
     int edge = 0;
     float d = 0.0f;
-    int len = walk_cells<WCSPH, false>(pi, cx, cy, cz, fR, pos_s, nullptr, n, g, P, nl + i, n, d, edge);
+    // each thread reads back only the LDS column it wrote, so no barrier is needed
+    const ListSink fl{stage + threadIdx.x, nl + i, n};
+    int len = walk_cells<WCSPH, false>(pi, cx, cy, cz, fR, pos_s, nullptr, n, g, P, fl, d, edge);
+    fl.flush(len);
     nc[i] = len;
     float bd = 0.0f;
     int blen = 0;
@@ -442,7 +465,11 @@ __global__ __launch_bounds__(256) void k_lists_density(int n, const float4* __re
             long long b = box_index(cx, cy, cz, g);
             skip = b >= 0 && bnear[b] == 0;
         }
-        if (!skip) blen = walk_cells<WCSPH, true>(pi, cx, cy, cz, bR, bpos_s, bphi_s, nb, g, P, bnl + i, n, bd, edge);
+        if (!skip) {
+            const ListSink bl{stage + threadIdx.x, bnl + i, n};
+            blen = walk_cells<WCSPH, true>(pi, cx, cy, cz, bR, bpos_s, bphi_s, nb, g, P, bl, bd, edge);
+            bl.flush(blen);
+        }
     }
     bnc[i] = blen;
     if (WCSPH) {
@@ -466,12 +493,12 @@ void launch_lists_density(bool wcsph, int n, const float4* pos_s, const int2* fR
                           float* pressure, float4* prho, DevState* st, hipStream_t s, Profiler* prof) {
     if (n <= 0) return;
     PROF_BEGIN(prof, "lists_density", s);
-    const int blocks = (n + 255) / 256;
+    const int blocks = (n + LISTS_THREADS - 1) / LISTS_THREADS;
     if (wcsph)
-        hipLaunchKernelGGL(k_lists_density<true>, dim3(blocks), dim3(256), 0, s, n, pos_s, fR, nb, bpos_s, bphi_s, bR,
+        hipLaunchKernelGGL(k_lists_density<true>, dim3(blocks), dim3(LISTS_THREADS), 0, s, n, pos_s, fR, nb, bpos_s, bphi_s, bR,
                            bnear, g, P, nl, nc, bnl, bnc, density, pressure, prho, st);
     else
-        hipLaunchKernelGGL(k_lists_density<false>, dim3(blocks), dim3(256), 0, s, n, pos_s, fR, nb, bpos_s, bphi_s, bR,
+        hipLaunchKernelGGL(k_lists_density<false>, dim3(blocks), dim3(LISTS_THREADS), 0, s, n, pos_s, fR, nb, bpos_s, bphi_s, bR,
                            bnear, g, P, nl, nc, bnl, bnc, density, pressure, prho, st);
     PROF_END(prof, s);
 }

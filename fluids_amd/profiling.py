@@ -1,0 +1,189 @@
+"""Summaries of rocprofv3 output for the solver's kernels.
+
+Two summaries are produced from rocprofv3's CSV output:
+
+* ``trace``: per-kernel dispatch count and average / min / max duration from
+  ``--kernel-trace`` (``*_kernel_trace.csv``), restricted to the dispatches at
+  each kernel's largest grid, i.e. the launches over the whole particle set
+  (the relaxation steps of a scene build run on fewer particles).
+* ``pmc``: per-kernel HBM traffic from two ``--pmc`` runs (FETCH_SIZE, then
+  WRITE_SIZE, in KiB per dispatch), each scaled by what the same counter reads
+  for a device copy of known size.
+
+Usage::
+
+    python -m fluids_amd.profiling copy --bytes 268435456   # calibration run, under --pmc
+    python -m fluids_amd.profiling trace <trace_dir> [--traffic traffic.md] > summary.md
+    python -m fluids_amd.profiling pmc <fetch_dir> <write_dir> <copy_fetch_dir> <copy_write_dir> \
+        --copy-bytes 268435456 [--tcc-dir <hit_miss_dir>] > traffic.md
+
+``bench.py`` reads the machine-readable ``traffic`` rows of a ``pmc`` summary
+(``read_traffic``) to fill its roofline's ``traffic`` field.
+"""
+import argparse
+import collections
+import csv
+import glob
+import os
+import re
+import sys
+
+# kernel symbol -> the name the solver's own profiler reports
+PROFILER_NAMES = {
+    "k_gather_fluid": "gather_ranges",
+}
+
+
+def short_name(kernel_name):
+    """'void sph::k_lists_density<false>(int, ...)' -> 'lists_density'."""
+    m = re.search(r"sph::(k_\w+)", kernel_name)
+    if not m:
+        return kernel_name.split("(")[0].strip()
+    sym = m.group(1)
+    return PROFILER_NAMES.get(sym, sym[2:])
+
+
+def _csv(path_or_dir, suffix):
+    if os.path.isdir(path_or_dir):
+        hits = sorted(glob.glob(os.path.join(path_or_dir, "**", f"*{suffix}"), recursive=True))
+        if not hits:
+            raise FileNotFoundError(f"no *{suffix} under {path_or_dir}")
+        return hits[0]
+    return path_or_dir
+
+
+def _grid(r):
+    # counter collection CSVs have Grid_Size, kernel traces Grid_Size_X/Y/Z
+    if "Grid_Size" in r:
+        return int(r["Grid_Size"])
+    return int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+
+
+def _full_grid(rows, key):
+    """Keeps each kernel's dispatches at its largest grid size."""
+    big = {}
+    for r in rows:
+        k = key(r)
+        big[k] = max(big.get(k, 0), _grid(r))
+    return [r for r in rows if _grid(r) == big[key(r)]]
+
+
+def trace_summary(trace_dir, traffic_file=None):
+    rows = list(csv.DictReader(open(_csv(trace_dir, "kernel_trace.csv"))))
+    rows = [r for r in rows if "sph::" in r["Kernel_Name"]]
+    rows = _full_grid(rows, lambda r: short_name(r["Kernel_Name"]))
+    dur = collections.defaultdict(list)
+    grid = {}
+    for r in rows:
+        k = short_name(r["Kernel_Name"])
+        dur[k].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+        grid[k] = _grid(r)
+    out = ["| kernel | grid (threads) | dispatches | avg us | min us | max us | total ms | HBM bytes | HBM GB/s |",
+           "|---|---|---|---|---|---|---|---|---|"]
+    for k, v in sorted(dur.items(), key=lambda kv: -sum(kv[1])):
+        avg = sum(v) / len(v)
+        t = read_traffic(traffic_file, k) if traffic_file else None
+        tb = "-" if t is None else f"{t:.0f}"
+        bw = "-" if t is None else f"{t / (avg * 1e-6) / 1e9:.0f}"
+        out.append(f"| {k} | {grid[k]} | {len(v)} | {avg:.1f} | {min(v):.1f} | {max(v):.1f} | "
+                   f"{sum(v) / 1e3:.3f} | {tb} | {bw} |")
+    return "\n".join(out)
+
+
+def _counter_means(d, counter):
+    rows = list(csv.DictReader(open(_csv(d, "counter_collection.csv"))))
+    rows = [r for r in rows if r["Counter_Name"] == counter]
+    return rows
+
+
+def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, copy_bytes, tcc_dir=None):
+    def per_kernel(d, counter):
+        rows = [r for r in _counter_means(d, counter) if "sph::" in r["Kernel_Name"]]
+        rows = _full_grid(rows, lambda r: short_name(r["Kernel_Name"]))
+        acc = collections.defaultdict(list)
+        for r in rows:
+            acc[short_name(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+        return {k: sum(v) / len(v) for k, v in acc.items()}, {k: len(v) for k, v in acc.items()}
+
+    def copy_kib(d, counter):
+        rows = [r for r in _counter_means(d, counter) if "copyBuffer" in r["Kernel_Name"]]
+        vals = sorted(float(r["Counter_Value"]) for r in rows)
+        return vals[len(vals) // 2]
+
+    f, nf = per_kernel(fetch_dir, "FETCH_SIZE")
+    w, nw = per_kernel(write_dir, "WRITE_SIZE")
+    hit = miss = None
+    if tcc_dir:
+        hit, _ = per_kernel(tcc_dir, "TCC_HIT_sum")
+        miss, _ = per_kernel(tcc_dir, "TCC_MISS_sum")
+    cf, cw = copy_kib(copy_fetch_dir, "FETCH_SIZE"), copy_kib(copy_write_dir, "WRITE_SIZE")
+    kf = copy_bytes / 1024.0 / cf
+    kw = copy_bytes / 1024.0 / cw
+    out = [f"Copy calibration: a {copy_bytes} B device copy reads FETCH_SIZE = {cf:.1f} KiB and "
+           f"WRITE_SIZE = {cw:.1f} KiB, so fetch KiB x {kf:.3f} and write KiB x {kw:.3f} give bytes.", "",
+           "| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | traffic bytes | L2 hit rate |",
+           "|---|---|---|---|---|---|"]
+    for k in sorted(set(f) & set(w), key=lambda k: -(f[k] * kf + w[k] * kw)):
+        t = 1024.0 * (f[k] * kf + w[k] * kw)
+        rate = "-"
+        if hit is not None and k in hit and hit[k] + miss.get(k, 0) > 0:
+            rate = f"{hit[k] / (hit[k] + miss[k]):.3f}"
+        out.append(f"| {k} | {nf[k]} | {f[k]:.1f} | {w[k]:.1f} | {t:.0f} | {rate} |")
+    out.append("")
+    out.append("Machine-readable rows (kernel, traffic bytes per dispatch):")
+    out.append("")
+    for k in sorted(set(f) & set(w)):
+        out.append(f"traffic {k} {1024.0 * (f[k] * kf + w[k] * kw):.0f}")
+    return "\n".join(out)
+
+
+def read_traffic(path, kernel):
+    """Traffic bytes per dispatch of `kernel` from a pmc summary, or None."""
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic" and parts[1] == kernel:
+            return float(parts[2])
+    return None
+
+
+def copy_calibration(nbytes):
+    """Three device-to-device copies of `nbytes`, the known-size kernel whose
+    FETCH_SIZE / WRITE_SIZE readings scale the solver's."""
+    import torch
+    a = torch.ones(nbytes // 4, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"copied {nbytes} B per launch", flush=True)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    c = sub.add_parser("copy")
+    c.add_argument("--bytes", type=int, default=1 << 28)
+    t = sub.add_parser("trace")
+    t.add_argument("trace_dir")
+    t.add_argument("--traffic", default=None, help="pmc summary whose traffic rows give HBM bytes per dispatch")
+    p = sub.add_parser("pmc")
+    p.add_argument("fetch_dir")
+    p.add_argument("write_dir")
+    p.add_argument("copy_fetch_dir")
+    p.add_argument("copy_write_dir")
+    p.add_argument("--copy-bytes", type=int, required=True)
+    p.add_argument("--tcc-dir", default=None, help="run with TCC_HIT_sum and TCC_MISS_sum, for L2 hit rates")
+    a = ap.parse_args(argv)
+    if a.cmd == "copy":
+        copy_calibration(a.bytes)
+    elif a.cmd == "trace":
+        print(trace_summary(a.trace_dir, a.traffic))
+    else:
+        print(pmc_summary(a.fetch_dir, a.write_dir, a.copy_fetch_dir, a.copy_write_dir, a.copy_bytes, a.tcc_dir))
+
+
+if __name__ == "__main__":
+    sys.exit(main())
