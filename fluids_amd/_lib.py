@@ -91,6 +91,18 @@ SIGNATURES = {
     "sph_sample_fishtank": (I, [F, F, F, F, P, I]),
     "sph_sample_cube": (I, [F, F, P, I]),
     "sph_sample_wall": (I, [F, F, F, P, I]),
+    "sph_rigid_create": (P, [F]),
+    "sph_rigid_destroy": (I, [P]),
+    "sph_rigid_set_container": (I, [P, F, F, F, P]),
+    "sph_rigid_add_box": (I, [P, F, P, P, P]),
+    "sph_rigid_add_sphere": (I, [P, F, F, P]),
+    "sph_rigid_add_static": (I, [P, P, P]),
+    "sph_rigid_apply_force_n_torque": (I, [P, I, P, P]),
+    "sph_rigid_step": (I, [P, F]),
+    "sph_rigid_get_state": (I, [P, I, P, P, P, P, P]),
+    "sph_rigid_set_state": (I, [P, I, P, P, P, P]),
+    "sph_rigid_body_count": (I, [P]),
+    "sph_coupled_step": (I, [P, P, I, P, P]),
 }
 
 _LIB = None

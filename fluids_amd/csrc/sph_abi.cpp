@@ -1,9 +1,11 @@
 // extern "C" entry points of libsph (declared in include/sph.h). Every entry
 // point converts exceptions into status codes; the message of the last
 // failure of the calling thread is kept for sph_last_error().
+#include <cmath>
 #include <cstring>
 #include <string>
 
+#include "rigid.h"
 #include "solver.h"
 #include "sph.h"
 
@@ -11,6 +13,11 @@ using namespace sph;
 
 struct sph_handle {
     std::unique_ptr<SimulationBase> sim;
+};
+
+struct sph_rigid_world {
+    RigidWorld world;
+    explicit sph_rigid_world(float g) : world(g) {}
 };
 
 namespace {
@@ -358,6 +365,121 @@ int sph_sample_wall(float width, float height, float spacing, float* out, int ca
     return guard([&] {
         if (!(spacing > 0.0f)) throw Error(SPH_ERR_INVALID, "spacing must be positive");
         return copy_samples(sample_wall(width, height, spacing), out, cap);
+    });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ rigid bodies
+namespace {
+RigidWorld& world_of(sph_rigid_world* w) {
+    if (!w) throw Error(SPH_ERR_INVALID, "rigid world is NULL");
+    return w->world;
+}
+}  // namespace
+
+extern "C" {
+
+sph_rigid_world* sph_rigid_create(float gravity_y) {
+    sph_rigid_world* out = nullptr;
+    int rc = guard([&] {
+        if (!std::isfinite(gravity_y)) throw Error(SPH_ERR_INVALID, "gravity must be finite");
+        out = new sph_rigid_world(gravity_y);
+        return SPH_OK;
+    });
+    return rc == SPH_OK ? out : nullptr;
+}
+
+int sph_rigid_destroy(sph_rigid_world* w) {
+    return guard([&] {
+        delete w;
+        return SPH_OK;
+    });
+}
+
+int sph_rigid_set_container(sph_rigid_world* w, float width, float height, float depth, const float center[3]) {
+    return guard([&] {
+        world_of(w).set_container(width, height, depth, center);
+        return SPH_OK;
+    });
+}
+
+int sph_rigid_add_box(sph_rigid_world* w, float mass, const float half[3], const float pos[3], const float quat[4]) {
+    return guard([&] { return world_of(w).add_box(mass, half, pos, quat); });
+}
+
+int sph_rigid_add_sphere(sph_rigid_world* w, float mass, float radius, const float pos[3]) {
+    return guard([&] { return world_of(w).add_sphere(mass, radius, pos); });
+}
+
+int sph_rigid_add_static(sph_rigid_world* w, const float pos[3], const float quat[4]) {
+    return guard([&] { return world_of(w).add_static(pos, quat); });
+}
+
+int sph_rigid_apply_force_n_torque(sph_rigid_world* w, int body, const float f[3], const float t[3]) {
+    return guard([&] {
+        world_of(w).apply_force_n_torque(body, f, t);
+        return SPH_OK;
+    });
+}
+
+int sph_rigid_step(sph_rigid_world* w, float dt) {
+    return guard([&] {
+        world_of(w).step(dt);
+        return SPH_OK;
+    });
+}
+
+int sph_rigid_get_state(sph_rigid_world* w, int body, float R[9], float com[3], float quat[4], float lin_vel[3],
+                        float ang_vel[3]) {
+    return guard([&] {
+        RigidWorld& W = world_of(w);
+        const RigidBody& b = W.body(body);
+        W.transform(body, R, com);
+        for (int k = 0; k < 4; ++k)
+            if (quat) quat[k] = b.q[k];
+        for (int k = 0; k < 3; ++k) {
+            if (lin_vel) lin_vel[k] = b.lin[k];
+            if (ang_vel) ang_vel[k] = b.ang[k];
+        }
+        return SPH_OK;
+    });
+}
+
+int sph_rigid_set_state(sph_rigid_world* w, int body, const float pos[3], const float quat[4], const float lin_vel[3],
+                        const float ang_vel[3]) {
+    return guard([&] {
+        RigidWorld& W = world_of(w);
+        W.set_pose(body, pos, quat);
+        W.set_velocity(body, lin_vel, ang_vel);
+        return SPH_OK;
+    });
+}
+
+int sph_rigid_body_count(sph_rigid_world* w) {
+    return guard([&] { return world_of(w).count(); });
+}
+
+int sph_coupled_step(sph_handle* h, sph_rigid_world* w, int npairs, const int* sph_ids, const int* rigid_ids) {
+    return guard([&] {
+        SimulationBase& S = sim_of(h);
+        RigidWorld& W = world_of(w);
+        if (npairs < 0 || (npairs > 0 && (!sph_ids || !rigid_ids))) throw Error(SPH_ERR_INVALID, "bad body pairs");
+        for (int k = 0; k < npairs; ++k) {
+            float R[9], com[3];
+            W.transform(rigid_ids[k], R, com);
+            const RigidBody& b = W.body(rigid_ids[k]);
+            S.set_body_state(sph_ids[k], R, com, b.lin, b.ang);
+        }
+        S.simulate();
+        for (int k = 0; k < npairs; ++k) {
+            if (!(W.body(rigid_ids[k]).mass > 0.0f)) continue;
+            float f[3], t[3];
+            S.body_wrench(sph_ids[k], f, t);
+            W.apply_force_n_torque(rigid_ids[k], f, t);
+        }
+        W.step(S.settings().time_step);
+        return SPH_OK;
     });
 }
 

@@ -253,6 +253,51 @@ int sph_sample_cube(float side, float spacing, float* out, int cap);
 /* Wall::surface_sampling (scene/wall.cpp:142-152), body-local. */
 int sph_sample_wall(float width, float height, float spacing, float* out, int cap);
 
+/* ------------------------------------------------------- rigid bodies (F1)
+ * A host rigid-body world in place of Bullet's btDiscreteDynamicsWorld for
+ * the two-way coupling loop (scene/scene.cpp:100-110, 304-316): gravity
+ * (0, -9.8, 0) as the reference hard-wires it, one variable step of dt per
+ * frame, contacts with the container's floor and walls and between bodies.
+ * Quaternions are (x, y, z, w) like btQuaternion and need not be unit. */
+typedef struct sph_rigid_world sph_rigid_world;
+
+/* btDiscreteDynamicsWorld + setGravity(0, gravity_y, 0) (scene.cpp:304-316). */
+sph_rigid_world* sph_rigid_create(float gravity_y);
+int sph_rigid_destroy(sph_rigid_world* w);
+/* FishTank's collision mesh (scene/fishtank.cpp:143-182): an open-top box of
+ * the given sides centred at center (NULL = origin). */
+int sph_rigid_set_container(sph_rigid_world* w, float width, float height, float depth, const float center[3]);
+/* Cube::_init_physics (scene/cube.cpp:129-134): a box body of the given
+ * half extents; mass 0 makes it static. Returns the body id. */
+int sph_rigid_add_box(sph_rigid_world* w, float mass, const float half[3], const float pos[3], const float quat[4]);
+/* Sphere's btSphereShape body (scene/sphere.cpp). Returns the body id. */
+int sph_rigid_add_sphere(sph_rigid_world* w, float mass, float radius, const float pos[3]);
+/* A body that never moves and collides with nothing (Wall, Model). */
+int sph_rigid_add_static(sph_rigid_world* w, const float pos[3], const float quat[4]);
+/* RigidBody::apply_force_n_torque (scene/rigidbody.cpp:49-54): clearForces,
+ * applyCentralForce(f), applyTorque(t). */
+int sph_rigid_apply_force_n_torque(sph_rigid_world* w, int body, const float f[3], const float t[3]);
+/* btDiscreteDynamicsWorld::stepSimulation(dt, 0) (scene.cpp:109). */
+int sph_rigid_step(sph_rigid_world* w, float dt);
+/* RigidBody::transform / position / rotation / linear_vel / angular_vel
+ * (scene/rigidbody.cpp:27-62): basis rows R, centre of mass, quaternion,
+ * velocities. Any output may be NULL. */
+int sph_rigid_get_state(sph_rigid_world* w, int body, float R[9], float com[3], float quat[4], float lin_vel[3],
+                        float ang_vel[3]);
+/* RigidBody::set_position / set_rotation, and velocities (NULL = keep). */
+int sph_rigid_set_state(sph_rigid_world* w, int body, const float pos[3], const float quat[4], const float lin_vel[3],
+                        const float ang_vel[3]);
+int sph_rigid_body_count(sph_rigid_world* w);
+
+/* One frame of Scene::render (scene/scene.cpp:100-110) for bodies that are
+ * both fluid boundaries and rigid bodies: for each pair k, the body state of
+ * rigid body rigid_ids[k] goes to fluid body sph_ids[k]
+ * (BoundaryHandler::sync, boundaryhandler.cpp:259-270); the fluid steps; each
+ * pair whose rigid body has mass > 0 gets the fluid's force and torque
+ * (apply_fluid_forces, boundaryhandler.cpp:322-346); then the rigid world
+ * steps by the fluid's time step. Synchronises the fluid's stream. */
+int sph_coupled_step(sph_handle* h, sph_rigid_world* w, int npairs, const int* sph_ids, const int* rigid_ids);
+
 #ifdef __cplusplus
 }
 #endif
