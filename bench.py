@@ -69,6 +69,48 @@ def run_cpu_baseline(sc, steps, threads):
     return sc.n_fluid * steps / dt, dt
 
 
+def run_batched(steps, warmup, plain_steps, count=512):
+    """Batched WCSPH: `count` seeded scenes (fluids_amd.scenes.c4_batched) in
+    one packed batch, HIP-graph steps, autotuned launch shapes; against the
+    same scenes as separate single-scene solvers stepped in turn (each on its
+    own stream, so their launches may overlap)."""
+    from fluids_amd import scenes
+    scs = scenes.c4_batched(count=count)
+    total = sum(sc.n_fluid for sc in scs)
+    t0 = time.perf_counter()
+    b = scenes.build_batch(scs, autotune=True, trials=2)
+    t_tune = time.perf_counter() - t0
+    b.step(warmup)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.synchronize()
+    wall = time.perf_counter() - t0
+    shapes = {name: b.launch(name) for name in ("batch_hash", "batch_gather_ranges", "batch_lists_density",
+                                                "batch_normals", "batch_accel_integrate")}
+    b.close()
+    sims = [scenes.build(sc) for sc in scs]
+    for sm in sims:
+        sm.step(1)
+    for sm in sims:
+        sm.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(plain_steps):
+        for sm in sims:
+            sm.step(1)
+    for sm in sims:
+        sm.synchronize()
+    wall_plain = time.perf_counter() - t1
+    for sm in sims:
+        sm.close()
+    v = total * steps / wall
+    vp = total * plain_steps / wall_plain
+    return {"workload": f"batched_wcsph_{count}", "scenes": count, "particles": total, "steps": steps,
+            "value": v, "unit": "particle-updates/s", "ms_per_step": 1e3 * wall / steps,
+            "plain_value": vp, "plain_steps": plain_steps, "plain_ms_per_step": 1e3 * wall_plain / plain_steps,
+            "speedup_vs_plain": v / vp, "launch_shapes": shapes, "build_and_tune_s": t_tune}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +122,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-events", action="store_true", help="time without per-kernel events")
     ap.add_argument("--json-out", default=None)
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-scenes measurement")
+    ap.add_argument("--batched-only", action="store_true", help="only the batched-scenes measurement")
+    ap.add_argument("--batched-steps", type=int, default=50)
+    ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "pmc_pcisph_throughput_1000k.md"),
                     help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary (python -m fluids_amd.profiling pmc) "
                          "that gives the dominant kernel's HBM bytes per launch")
@@ -93,6 +139,9 @@ def main():
     from fluids_amd import scenes
     from fluids_amd._lib import lib
     lib()
+    if a.batched_only:
+        print(json.dumps({"batched": run_batched(a.batched_steps, 5, a.plain_steps)}), flush=True)
+        return
     t_init = time.perf_counter()
     sc = scenes.c3_pcisph_throughput(a.n)
     if a.max_iter is not None:
@@ -162,6 +211,10 @@ def main():
                "sample": f"{a.cpu_steps} PCISPH steps of the same {N}-particle scene after its relaxation step "
                          f"({dt:.1f} s), fp32 oracle, OpenMP"}
 
+    batched = None
+    if not a.no_batched:
+        batched = run_batched(a.batched_steps, 5, a.plain_steps)
+
     rec = {
         "metric": "particle-updates/s (PCISPH 1M fp32), %HBM roofline; batched & adjoint vs plain",
         "value": pu_s, "unit": "particle-updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -173,6 +226,7 @@ def main():
         "roofline": roof,
         "step_roofline": {"alg_bytes_per_pu": bpp, "achieved_GBps": bpp * pu_s / 1e9, "frac": bpp * pu_s / HBM_PEAK},
         "cpu_baseline": cpu,
+        "batched": batched,
         "kernels": per,
         "events_in_timed_region": not a.no_events,
     }

@@ -103,6 +103,28 @@ SIGNATURES = {
     "sph_rigid_set_state": (I, [P, I, P, P, P, P]),
     "sph_rigid_body_count": (I, [P]),
     "sph_coupled_step": (I, [P, P, I, P, P]),
+    "sph_batch_create": (P, []),
+    "sph_batch_destroy": (I, [P]),
+    "sph_batch_add_scene": (I, [P, C.POINTER(SphSettings), F, F, F, P, I]),
+    "sph_batch_finalize": (I, [P]),
+    "sph_batch_autotune": (I, [P, C.c_char_p, I]),
+    "sph_batch_get_launch": (I, [P, I, P, P, P]),
+    "sph_batch_set_launch": (I, [P, I, I, I, I]),
+    "sph_batch_set_graph": (I, [P, I]),
+    "sph_batch_step": (I, [P, I]),
+    "sph_batch_synchronize": (I, [P]),
+    "sph_batch_scene_count": (I, [P]),
+    "sph_batch_particle_count": (I, [P, I]),
+    "sph_batch_total_particles": (I, [P]),
+    "sph_batch_get_positions": (I, [P, I, P, I]),
+    "sph_batch_get_velocities": (I, [P, I, P, I]),
+    "sph_batch_get_half_velocities": (I, [P, I, P, I]),
+    "sph_batch_set_positions": (I, [P, I, P, I]),
+    "sph_batch_set_velocities": (I, [P, I, P, I]),
+    "sph_batch_get_density": (I, [P, I, P, I]),
+    "sph_batch_get_neighbour_counts": (I, [P, I, P, I]),
+    "sph_batch_set_profiling": (I, [P, I, C.c_char_p]),
+    "sph_batch_get_kernel_times": (I, [P, P, P, P, I]),
 }
 
 _LIB = None

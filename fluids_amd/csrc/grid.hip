@@ -7,6 +7,9 @@
 // (stable key/value sort), opencl/algorithms/clshuffle.h:23-35 (gather) and
 // the density kernels that consume the lists (pcisph.cl:33-102,
 // wcsph.cl:32-79).
+#include <stdexcept>
+
+#include "batch.h"
 #include "kernels.h"
 #include "sph_device.h"
 
@@ -17,11 +20,8 @@ namespace sph {
 // identity; the first sort pass generates it instead of reading it. Before
 // writing the new key, each thread clears the cell range named by the
 // previous step's sorted key at its index (prev may alias keys).
-__global__ __launch_bounds__(256) void k_hash(int n, const float4* __restrict__ pos, GridDev g,
-                                              uint32_t* keys, const uint32_t* prev, int2* __restrict__ R,
-                                              DevState* st) {
-    int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void hash_one(int i, const float4* __restrict__ pos, const GridDev& g, uint32_t* keys,
+                                         const uint32_t* prev, int2* __restrict__ R, DevState* st) {
     if (R) {
         uint32_t pk = prev[i];
         int px = (int)compact_by_3(pk), py = (int)compact_by_3(pk >> 1), pz = (int)compact_by_3(pk >> 2);
@@ -34,6 +34,14 @@ __global__ __launch_bounds__(256) void k_hash(int n, const float4* __restrict__ 
     grid_coords(p, g, cx, cy, cz);
     keys[i] = morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
     if (box_index(cx, cy, cz, g) < 0) atomicAdd(&st->out_of_grid, 1);
+}
+
+__global__ __launch_bounds__(256) void k_hash(int n, const float4* __restrict__ pos, GridDev g,
+                                              uint32_t* keys, const uint32_t* prev, int2* __restrict__ R,
+                                              DevState* st) {
+    int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    hash_one(i, pos, g, keys, prev, R, st);
 }
 
 // ------------------------------------------------------------- radix sort
@@ -65,9 +73,20 @@ __device__ __forceinline__ uint32_t block_incl_scan_256(uint32_t v, uint32_t* wt
     return v + add;
 }
 
-template <int BITS>
-__global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint32_t* __restrict__ keys, int n, int shift,
-                                                             int ntiles, uint32_t* __restrict__ hist) {
+// With SID the digit is the scene id of the value (sid[value]) instead of a
+// key digit: one such pass after the key passes sorts a packed batch of
+// scenes by (scene, key), i.e. each scene's segment by its own keys.
+template <int BITS, bool SID>
+__device__ __forceinline__ uint32_t radix_digit(uint32_t key, int val, int shift, const int* __restrict__ sid) {
+    if (SID) return (uint32_t)sid[val];
+    return (key >> shift) & ((1u << BITS) - 1u);
+}
+
+template <int BITS, bool SID>
+__global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint32_t* __restrict__ keys,
+                                                             const int* __restrict__ vin, int n, int shift,
+                                                             int ntiles, uint32_t* __restrict__ hist,
+                                                             const int* __restrict__ sid) {
     constexpr int RADIX = 1 << BITS;
     __shared__ uint32_t h[RADIX];
     const int t = threadIdx.x;
@@ -77,7 +96,10 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint32_t* __r
 #pragma unroll 4
     for (int k = 0; k < SORT_ITEMS; ++k) {
         int idx = base + k * SORT_THREADS + t;
-        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & (RADIX - 1)], 1u);
+        if (idx < n) {
+            int val = SID ? (vin ? vin[idx] : idx) : 0;
+            atomicAdd(&h[radix_digit<BITS, SID>(SID ? 0u : keys[idx], val, shift, sid)], 1u);
+        }
     }
     __syncthreads();
     for (int d = t; d < RADIX; d += SORT_THREADS) hist[(size_t)d * ntiles + blockIdx.x] = h[d];
@@ -102,13 +124,14 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_rowscan(uint32_t* __rest
     if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
-template <int BITS>
+template <int BITS, bool SID>
 __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint32_t* __restrict__ kin,
                                                                 const int* __restrict__ vin,
                                                                 uint32_t* __restrict__ kout, int* __restrict__ vout,
                                                                 int n, int shift, int ntiles,
                                                                 const uint32_t* __restrict__ offs,
-                                                                const uint32_t* __restrict__ totals) {
+                                                                const uint32_t* __restrict__ totals,
+                                                                const int* __restrict__ sid) {
     constexpr int RADIX = 1 << BITS;
     constexpr int PER = RADIX / SORT_THREADS;  // digits per thread for the table set-up
     constexpr int WAVES = SORT_THREADS / 64;
@@ -139,7 +162,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint32_t* 
         bool valid = idx < n;
         uint32_t key = valid ? kin[idx] : 0u;
         int val = valid ? (vin ? vin[idx] : idx) : 0;
-        uint32_t d = (key >> shift) & (RADIX - 1);
+        uint32_t d = valid ? radix_digit<BITS, SID>(key, val, shift, sid) : 0u;
         uint64_t m = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < BITS; ++b) {
@@ -197,39 +220,54 @@ size_t sort_hist_words(int n) {
     return (size_t)RADIX_MAX * (ntiles > 0 ? ntiles : 1) + RADIX_MAX;
 }
 
-template <int BITS>
+template <int BITS, bool SID>
 static void radix_pass(const uint32_t* kin, const int* vin, uint32_t* kout, int* vout, uint32_t* hist,
-                       uint32_t* totals, int n, int ntiles, int shift, hipStream_t s, Profiler* prof) {
+                       uint32_t* totals, int n, int ntiles, int shift, const int* sid, hipStream_t s,
+                       Profiler* prof) {
     PROF_BEGIN(prof, "radix_hist", s);
-    hipLaunchKernelGGL(k_radix_hist<BITS>, dim3(ntiles), dim3(SORT_THREADS), 0, s, kin, n, shift, ntiles, hist);
+    hipLaunchKernelGGL((k_radix_hist<BITS, SID>), dim3(ntiles), dim3(SORT_THREADS), 0, s, kin, vin, n, shift, ntiles,
+                       hist, sid);
     PROF_END(prof, s);
     PROF_BEGIN(prof, "radix_rowscan", s);
     hipLaunchKernelGGL(k_radix_rowscan, dim3(1 << BITS), dim3(SORT_THREADS), 0, s, hist, ntiles, totals);
     PROF_END(prof, s);
     PROF_BEGIN(prof, "radix_scatter", s);
-    hipLaunchKernelGGL(k_radix_scatter<BITS>, dim3(ntiles), dim3(SORT_THREADS), 0, s, kin, vin, kout, vout, n, shift,
-                       ntiles, hist, totals);
+    hipLaunchKernelGGL((k_radix_scatter<BITS, SID>), dim3(ntiles), dim3(SORT_THREADS), 0, s, kin, vin, kout, vout, n,
+                       shift, ntiles, hist, totals, sid);
     PROF_END(prof, s);
 }
 
+int sort_passes(int key_bits, int sid_bits) { return sort_passes(key_bits) + (sid_bits > 0 ? 1 : 0); }
+
 void launch_radix_sort(uint32_t* keys_a, uint32_t* keys_b, int* vals_a, int* vals_b, uint32_t* hist, int n,
-                       int key_bits, hipStream_t s, Profiler* prof) {
+                       int key_bits, hipStream_t s, Profiler* prof, const int* sid, int sid_bits) {
     if (n <= 0) return;
     int passes;
     const int bits = digit_bits(key_bits, passes);
+    if (sid && sid_bits > 10) throw std::invalid_argument("radix sort: more than 1024 segments");
+    const int total = passes + (sid ? 1 : 0);
     int ntiles = (n + SORT_TILE - 1) / SORT_TILE;
     uint32_t* totals = hist + (size_t)RADIX_MAX * ntiles;
-    // The caller hashed into keys_b when passes is odd, keys_a when even;
-    // either way the result lands in (keys_a, vals_a).
-    uint32_t* kin = (passes % 2) ? keys_b : keys_a;
-    uint32_t* kout = (passes % 2) ? keys_a : keys_b;
+    // The caller hashed into keys_b when the pass count is odd, keys_a when
+    // even; either way the result lands in (keys_a, vals_a).
+    uint32_t* kin = (total % 2) ? keys_b : keys_a;
+    uint32_t* kout = (total % 2) ? keys_a : keys_b;
     int* vin = nullptr;
-    int* vout = (passes % 2) ? vals_a : vals_b;
-    for (int p = 0; p < passes; ++p) {
+    int* vout = (total % 2) ? vals_a : vals_b;
+    for (int p = 0; p < total; ++p) {
+        const bool scene_pass = sid && p == passes;
         int shift = bits * p;
-        if (bits == 8) radix_pass<8>(kin, vin, kout, vout, hist, totals, n, ntiles, shift, s, prof);
-        else if (bits == 9) radix_pass<9>(kin, vin, kout, vout, hist, totals, n, ntiles, shift, s, prof);
-        else radix_pass<10>(kin, vin, kout, vout, hist, totals, n, ntiles, shift, s, prof);
+        if (scene_pass) {
+            if (sid_bits <= 8) radix_pass<8, true>(kin, vin, kout, vout, hist, totals, n, ntiles, 0, sid, s, prof);
+            else if (sid_bits == 9) radix_pass<9, true>(kin, vin, kout, vout, hist, totals, n, ntiles, 0, sid, s, prof);
+            else radix_pass<10, true>(kin, vin, kout, vout, hist, totals, n, ntiles, 0, sid, s, prof);
+        } else if (bits == 8) {
+            radix_pass<8, false>(kin, vin, kout, vout, hist, totals, n, ntiles, shift, nullptr, s, prof);
+        } else if (bits == 9) {
+            radix_pass<9, false>(kin, vin, kout, vout, hist, totals, n, ntiles, shift, nullptr, s, prof);
+        } else {
+            radix_pass<10, false>(kin, vin, kout, vout, hist, totals, n, ntiles, shift, nullptr, s, prof);
+        }
         uint32_t* kt = kin; kin = kout; kout = kt;
         int* nv = (vout == vals_a) ? vals_b : vals_a;
         vin = vout;
@@ -250,11 +288,11 @@ void launch_hash(int n, const float4* pos, const GridDev& g, uint32_t* keys, con
 // wcsphsimluation.cpp:57-59) fused with compute_cell_intervals
 // (grid.cl:60-123): a cell's range starts where the sorted key changes and
 // ends before the next change.
-__device__ __forceinline__ void write_range(int i, int n, const uint32_t* __restrict__ keys_s, float4 p,
+__device__ __forceinline__ void write_range(int i, int lo, int hi, const uint32_t* __restrict__ keys_s, float4 p,
                                             int2* __restrict__ R, const GridDev& g) {
     uint32_t key = keys_s[i];
-    bool first = (i == 0) || keys_s[i - 1] != key;
-    bool last = (i == n - 1) || keys_s[i + 1] != key;
+    bool first = (i == lo) || keys_s[i - 1] != key;
+    bool last = (i == hi - 1) || keys_s[i + 1] != key;
     if (first || last) {
         int cx, cy, cz;
         grid_coords(p, g, cx, cy, cz);
@@ -266,6 +304,21 @@ __device__ __forceinline__ void write_range(int i, int n, const uint32_t* __rest
     }
 }
 
+// sorted index i of the segment [lo, hi) that holds one simulation
+__device__ __forceinline__ void gather_one(int i, int lo, int hi, const int* __restrict__ perm,
+                                           const uint32_t* __restrict__ keys_s, const float4* __restrict__ pos,
+                                           const float4* __restrict__ vel, const float4* __restrict__ vh,
+                                           float4* __restrict__ pos_s, float4* __restrict__ vel_s,
+                                           float4* __restrict__ vh_s, int2* __restrict__ R, const GridDev& g) {
+    int j = perm[i];
+    if (j < lo || j >= hi) j = i;  // a permutation never leaves its segment
+    float4 p = pos[j];
+    pos_s[i] = p;
+    vel_s[i] = vel[j];
+    if (vh) vh_s[i] = vh[j];
+    write_range(i, lo, hi, keys_s, p, R, g);
+}
+
 __global__ __launch_bounds__(256) void k_gather_fluid(int n, const int* __restrict__ perm,
                                                       const uint32_t* __restrict__ keys_s,
                                                       const float4* __restrict__ pos, const float4* __restrict__ vel,
@@ -274,13 +327,7 @@ __global__ __launch_bounds__(256) void k_gather_fluid(int n, const int* __restri
                                                       int2* __restrict__ R, GridDev g) {
     int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int j = perm[i];
-    if ((unsigned)j >= (unsigned)n) j = i;  // a permutation never leaves [0, n)
-    float4 p = pos[j];
-    pos_s[i] = p;
-    vel_s[i] = vel[j];
-    if (vh) vh_s[i] = vh[j];
-    write_range(i, n, keys_s, p, R, g);
+    gather_one(i, 0, n, perm, keys_s, pos, vel, vh, pos_s, vel_s, vh_s, R, g);
 }
 
 __global__ __launch_bounds__(256) void k_gather_boundary(int n, const int* __restrict__ perm,
@@ -295,7 +342,7 @@ __global__ __launch_bounds__(256) void k_gather_boundary(int n, const int* __res
     float4 p = pos[j];
     pos_s[i] = p;
     phi_s[i] = phi[j];
-    write_range(i, n, keys_s, p, R, g);
+    write_range(i, 0, n, keys_s, p, R, g);
 }
 
 void launch_gather_fluid(int n, const int* perm, const uint32_t* keys_s, const float4* pos, const float4* vel,
@@ -377,27 +424,28 @@ constexpr int LISTS_THREADS = 64;
 // one at a time, 64 consecutive entries per wave.
 constexpr int STAGE_SLOTS = 24;
 
+template <int TPB, int SLOTS>
 struct ListSink {
-    int* stage;  // this thread's column of the LDS stage, stride LISTS_THREADS
+    int* stage;  // this thread's column of the LDS stage, stride TPB
     int* out;    // list + i, stride n
     int n;
     __device__ __forceinline__ void put(int k, int j) const {
-        if (k < STAGE_SLOTS)
-            stage[k * LISTS_THREADS] = j;
+        if (k < SLOTS)
+            stage[k * TPB] = j;
         else
             out[(size_t)k * n] = j;
     }
     __device__ __forceinline__ void flush(int len) const {
-        const int m = len < STAGE_SLOTS ? len : STAGE_SLOTS;
-        for (int k = 0; k < m; ++k) out[(size_t)k * n] = stage[k * LISTS_THREADS];
+        const int m = len < SLOTS ? len : SLOTS;
+        for (int k = 0; k < m; ++k) out[(size_t)k * n] = stage[k * TPB];
     }
 };
 
-template <bool WCSPH, bool PHI>
+template <bool WCSPH, bool PHI, class Sink>
 __device__ __forceinline__ int walk_cells(V3 pi, int cx, int cy, int cz, const int2* __restrict__ R,
                                           const float4* __restrict__ P4, const float* __restrict__ phi,
                                           int count, const GridDev& g, const DensityParams& P,
-                                          const ListSink& list, float& d, int& edge) {
+                                          const Sink& list, float& d, int& edge) {
     int len = 0;
     int2 cur = cell_range(R, cx, cy, cz, g, edge);  // own cell first
     for (int o = 0; o < 27 && len < LIST_CAP; ++o) {
@@ -437,15 +485,16 @@ __device__ __forceinline__ int walk_cells(V3 pi, int cx, int cy, int cz, const i
     return len;
 }
 
-template <bool WCSPH>
-__global__ __launch_bounds__(LISTS_THREADS) void k_lists_density(
-    int n, const float4* __restrict__ pos_s, const int2* __restrict__ fR, int nb, const float4* __restrict__ bpos_s,
-    const float* __restrict__ bphi_s, const int2* __restrict__ bR, const uint8_t* __restrict__ bnear, GridDev g,
-    DensityParams P, int* __restrict__ nl, int* __restrict__ nc, int* __restrict__ bnl, int* __restrict__ bnc,
-    float* __restrict__ density, float* __restrict__ pressure, float4* __restrict__ prho, DevState* st) {
-    __shared__ int stage[STAGE_SLOTS * LISTS_THREADS];
-    int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// One particle of the lists + density pass. `count` bounds the indices a
+// range may name (the end of the particle's segment); `stride` is the list
+// stride (the total particle count).
+template <bool WCSPH, int TPB = LISTS_THREADS, int SLOTS = STAGE_SLOTS>
+__device__ __forceinline__ void lists_density_one(
+    int i, int stride, int count, const float4* __restrict__ pos_s, const int2* __restrict__ fR, int nb,
+    const float4* __restrict__ bpos_s, const float* __restrict__ bphi_s, const int2* __restrict__ bR,
+    const uint8_t* __restrict__ bnear, const GridDev& g, const DensityParams& P, int* __restrict__ nl,
+    int* __restrict__ nc, int* __restrict__ bnl, int* __restrict__ bnc, float* __restrict__ density,
+    float* __restrict__ pressure, float4* __restrict__ prho, DevState* st, int* stage_col) {
     float4 p4 = pos_s[i];
     V3 pi = v3(p4);
     int cx, cy, cz;
@@ -453,8 +502,8 @@ __global__ __launch_bounds__(LISTS_THREADS) void k_lists_density(
     int edge = 0;
     float d = 0.0f;
     // each thread reads back only the LDS column it wrote, so no barrier is needed
-    const ListSink fl{stage + threadIdx.x, nl + i, n};
-    int len = walk_cells<WCSPH, false>(pi, cx, cy, cz, fR, pos_s, nullptr, n, g, P, fl, d, edge);
+    const ListSink<TPB, SLOTS> fl{stage_col, nl + i, stride};
+    int len = walk_cells<WCSPH, false>(pi, cx, cy, cz, fR, pos_s, nullptr, count, g, P, fl, d, edge);
     fl.flush(len);
     nc[i] = len;
     float bd = 0.0f;
@@ -466,7 +515,7 @@ __global__ __launch_bounds__(LISTS_THREADS) void k_lists_density(
             skip = b >= 0 && bnear[b] == 0;
         }
         if (!skip) {
-            const ListSink bl{stage + threadIdx.x, bnl + i, n};
+            const ListSink<TPB, SLOTS> bl{stage_col, bnl + i, stride};
             blen = walk_cells<WCSPH, true>(pi, cx, cy, cz, bR, bpos_s, bphi_s, nb, g, P, bl, bd, edge);
             bl.flush(blen);
         }
@@ -487,6 +536,19 @@ __global__ __launch_bounds__(LISTS_THREADS) void k_lists_density(
     if (edge) atomicAdd(&st->edge_reads, edge);
 }
 
+template <bool WCSPH>
+__global__ __launch_bounds__(LISTS_THREADS) void k_lists_density(
+    int n, const float4* __restrict__ pos_s, const int2* __restrict__ fR, int nb, const float4* __restrict__ bpos_s,
+    const float* __restrict__ bphi_s, const int2* __restrict__ bR, const uint8_t* __restrict__ bnear, GridDev g,
+    DensityParams P, int* __restrict__ nl, int* __restrict__ nc, int* __restrict__ bnl, int* __restrict__ bnc,
+    float* __restrict__ density, float* __restrict__ pressure, float4* __restrict__ prho, DevState* st) {
+    __shared__ int stage[STAGE_SLOTS * LISTS_THREADS];
+    int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    lists_density_one<WCSPH>(i, n, n, pos_s, fR, nb, bpos_s, bphi_s, bR, bnear, g, P, nl, nc, bnl, bnc, density,
+                             pressure, prho, st, stage + threadIdx.x);
+}
+
 void launch_lists_density(bool wcsph, int n, const float4* pos_s, const int2* fR, int nb, const float4* bpos_s,
                           const float* bphi_s, const int2* bR, const uint8_t* bnear, const GridDev& g,
                           const DensityParams& P, int* nl, int* nc, int* bnl, int* bnc, float* density,
@@ -505,11 +567,9 @@ void launch_lists_density(bool wcsph, int n, const float4* pos_s, const int2* fR
 
 // ------------------------------------------------------------------ normals
 // compute_normals (normals.cl:23-70); prho holds (x, y, z, density).
-__global__ __launch_bounds__(256) void k_normals(int n, const float4* __restrict__ prho,
-                                                 const int* __restrict__ nl, const int* __restrict__ nc, float H2,
-                                                 float w_grad, float M, float H, float4* __restrict__ normal) {
-    int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void normals_one(int i, int n, const float4* __restrict__ prho,
+                                            const int* __restrict__ nl, const int* __restrict__ nc, float H2,
+                                            float w_grad, float M, float H, float4* __restrict__ normal) {
     V3 pi = v3(prho[i]);
     V3 acc = {0.0f, 0.0f, 0.0f};
     const int len = nc[i];
@@ -534,6 +594,14 @@ __global__ __launch_bounds__(256) void k_normals(int n, const float4* __restrict
     o = o * M;
     o = o * H;
     normal[i] = f4(o, 0.0f);
+}
+
+__global__ __launch_bounds__(256) void k_normals(int n, const float4* __restrict__ prho,
+                                                 const int* __restrict__ nl, const int* __restrict__ nc, float H2,
+                                                 float w_grad, float M, float H, float4* __restrict__ normal) {
+    int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    normals_one(i, n, prho, nl, nc, H2, w_grad, M, H, normal);
 }
 
 void launch_normals(int n, const float4* prho, const int* nl, const int* nc, float H2, float w_grad, float M,
@@ -609,6 +677,126 @@ void launch_stats(int n, const int* nc, const int* bnc, const float4* pos, unsig
     launch_fill_u32(acc, 20, 0u, s);
     if (n <= 0) return;
     hipLaunchKernelGGL(k_stats, dim3((n + 255) / 256), dim3(256), 0, s, n, nc, bnc, pos, acc);
+}
+
+
+// ------------------------------------------------------------ batched scenes
+// Kernels over a packed batch of scenes (batch.h): each particle looks up its
+// scene's grid and parameters and runs the same per-particle function as the
+// single-scene kernel. A grid smaller than one thread per particle strides
+// over the particles (persistent blocks).
+#define BATCH_LOOP(n)                                                                         \
+    for (int base_ = xcd_block(blockIdx.x, gridDim.x) * blockDim.x; base_ < (n);             \
+         base_ += gridDim.x * blockDim.x)                                                     \
+        for (int i = base_ + threadIdx.x; i < (n); i = (n))
+
+__global__ __launch_bounds__(256) void k_batch_hash(int n, const int* __restrict__ sid,
+                                                    const BatchScene* __restrict__ sc,
+                                                    const float4* __restrict__ pos, uint32_t* keys,
+                                                    const uint32_t* prev, int2* __restrict__ R, int clear,
+                                                    DevState* st) {
+    BATCH_LOOP(n) {
+        const BatchScene& S = sc[sid[i]];
+        hash_one(i, pos, S.g, keys, prev, clear ? R + S.roff : nullptr, st);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_batch_gather(int n, const int* __restrict__ sid,
+                                                      const BatchScene* __restrict__ sc, const int* __restrict__ perm,
+                                                      const uint32_t* __restrict__ keys_s,
+                                                      const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                                      const float4* __restrict__ vh, float4* __restrict__ pos_s,
+                                                      float4* __restrict__ vel_s, float4* __restrict__ vh_s,
+                                                      int2* __restrict__ R) {
+    BATCH_LOOP(n) {
+        const BatchScene& S = sc[sid[i]];
+        gather_one(i, S.lo, S.hi, perm, keys_s, pos, vel, vh, pos_s, vel_s, vh_s, R + S.roff, S.g);
+    }
+}
+
+template <int TPB, int SLOTS>
+__global__ __launch_bounds__(TPB) void k_batch_lists_density(int n, const int* __restrict__ sid,
+                                                             const BatchScene* __restrict__ sc,
+                                                             const float4* __restrict__ pos_s,
+                                                             const int2* __restrict__ R, int* __restrict__ nl,
+                                                             int* __restrict__ nc, int* __restrict__ bnc,
+                                                             float* __restrict__ density,
+                                                             float* __restrict__ pressure, float4* __restrict__ prho,
+                                                             DevState* st) {
+    __shared__ int stage[SLOTS > 0 ? SLOTS * TPB : 1];
+    BATCH_LOOP(n) {
+        const BatchScene& S = sc[sid[i]];
+        lists_density_one<true, TPB, SLOTS>(i, n, S.hi, pos_s, R + S.roff, 0, nullptr, nullptr, nullptr, nullptr,
+                                            S.g, S.D, nl, nc, nullptr, bnc, density, pressure, prho, st,
+                                            stage + threadIdx.x);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_batch_normals(int n, const int* __restrict__ sid,
+                                                       const BatchScene* __restrict__ sc,
+                                                       const float4* __restrict__ prho, const int* __restrict__ nl,
+                                                       const int* __restrict__ nc, float4* __restrict__ normal) {
+    BATCH_LOOP(n) {
+        const BatchScene& S = sc[sid[i]];
+        normals_one(i, n, prho, nl, nc, S.nH2, S.nw_grad, S.nM, S.nH, normal);
+    }
+}
+
+void launch_batch_hash(int n, const int* sid, const BatchScene* sc, const float4* pos, uint32_t* keys,
+                       const uint32_t* prev, int2* R, bool clear, DevState* st, const LaunchCfg& c, hipStream_t s,
+                       Profiler* prof) {
+    if (n <= 0) return;
+    PROF_BEGIN(prof, "batch_hash", s);
+    hipLaunchKernelGGL(k_batch_hash, dim3(batch_blocks(n, c)), dim3(c.block), 0, s, n, sid, sc, pos, keys, prev, R,
+                       clear ? 1 : 0, st);
+    PROF_END(prof, s);
+}
+
+void launch_batch_gather(int n, const int* sid, const BatchScene* sc, const int* perm, const uint32_t* keys_s,
+                         const float4* pos, const float4* vel, const float4* vh, float4* pos_s, float4* vel_s,
+                         float4* vh_s, int2* R, const LaunchCfg& c, hipStream_t s, Profiler* prof) {
+    if (n <= 0) return;
+    PROF_BEGIN(prof, "batch_gather_ranges", s);
+    hipLaunchKernelGGL(k_batch_gather, dim3(batch_blocks(n, c)), dim3(c.block), 0, s, n, sid, sc, perm, keys_s, pos,
+                       vel, vh, pos_s, vel_s, vh_s, R);
+    PROF_END(prof, s);
+}
+
+template <int TPB, int SLOTS>
+static void lists_variant(int n, const int* sid, const BatchScene* sc, const float4* pos_s, const int2* R, int* nl,
+                          int* nc, int* bnc, float* density, float* pressure, float4* prho, DevState* st,
+                          const LaunchCfg& c, hipStream_t s) {
+    hipLaunchKernelGGL((k_batch_lists_density<TPB, SLOTS>), dim3(batch_blocks(n, c)), dim3(TPB), 0, s, n, sid, sc,
+                       pos_s, R, nl, nc, bnc, density, pressure, prho, st);
+}
+
+void launch_batch_lists_density(int n, const int* sid, const BatchScene* sc, const float4* pos_s, const int2* R,
+                                int* nl, int* nc, int* bnc, float* density, float* pressure, float4* prho,
+                                DevState* st, const LaunchCfg& c, hipStream_t s, Profiler* prof) {
+    if (n <= 0) return;
+    PROF_BEGIN(prof, "batch_lists_density", s);
+    const bool staged = c.stage > 0;
+    if (c.block == 64)
+        staged ? lists_variant<64, 24>(n, sid, sc, pos_s, R, nl, nc, bnc, density, pressure, prho, st, c, s)
+               : lists_variant<64, 0>(n, sid, sc, pos_s, R, nl, nc, bnc, density, pressure, prho, st, c, s);
+    else if (c.block == 128)
+        staged ? lists_variant<128, 24>(n, sid, sc, pos_s, R, nl, nc, bnc, density, pressure, prho, st, c, s)
+               : lists_variant<128, 0>(n, sid, sc, pos_s, R, nl, nc, bnc, density, pressure, prho, st, c, s);
+    else if (c.block == 256)
+        staged ? lists_variant<256, 24>(n, sid, sc, pos_s, R, nl, nc, bnc, density, pressure, prho, st, c, s)
+               : lists_variant<256, 0>(n, sid, sc, pos_s, R, nl, nc, bnc, density, pressure, prho, st, c, s);
+    else
+        throw std::invalid_argument("batched lists: block must be 64, 128 or 256");
+    PROF_END(prof, s);
+}
+
+void launch_batch_normals(int n, const int* sid, const BatchScene* sc, const float4* prho, const int* nl,
+                          const int* nc, float4* normal, const LaunchCfg& c, hipStream_t s, Profiler* prof) {
+    if (n <= 0) return;
+    PROF_BEGIN(prof, "batch_normals", s);
+    hipLaunchKernelGGL(k_batch_normals, dim3(batch_blocks(n, c)), dim3(c.block), 0, s, n, sid, sc, prho, nl, nc,
+                       normal);
+    PROF_END(prof, s);
 }
 
 }  // namespace sph

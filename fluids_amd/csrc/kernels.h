@@ -89,11 +89,16 @@ struct BodyDev {
 
 // grid.hip
 int sort_passes(int key_bits);
+// passes of a sort whose last pass orders by scene id (sid_bits > 0)
+int sort_passes(int key_bits, int sid_bits);
 size_t sort_hist_words(int n);
 void launch_hash(int n, const float4* pos, const GridDev& g, uint32_t* keys, const uint32_t* prev, int2* R,
                  DevState* st, hipStream_t s, Profiler* prof);
+// Stable sort of n (key, index) pairs by the low key_bits of the keys; with
+// sid (scene id of each index, < 2^sid_bits) a final pass orders the result
+// by scene, so each scene's contiguous segment is sorted by its own keys.
 void launch_radix_sort(uint32_t* keys_a, uint32_t* keys_b, int* vals_a, int* vals_b, uint32_t* hist, int n,
-                       int key_bits, hipStream_t s, Profiler* prof);
+                       int key_bits, hipStream_t s, Profiler* prof, const int* sid = nullptr, int sid_bits = 0);
 void launch_gather_fluid(int n, const int* perm, const uint32_t* keys_s, const float4* pos, const float4* vel,
                          const float4* vh, float4* pos_s, float4* vel_s, float4* vh_s, int2* R, const GridDev& g,
                          hipStream_t s, Profiler* prof);

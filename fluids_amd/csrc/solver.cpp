@@ -893,14 +893,57 @@ WCSPHSimulation::WCSPHSimulation(const Settings& s) : SimulationBase(s) { initia
 
 // _initialize_params (wcsphsimluation.cpp:165-202)
 void WCSPHSimulation::initialize_params() {
-    float r = s_.fluid_particle_radius, h = s_.fluid_support_radius;
+    float r = s_.fluid_particle_radius;
     mass_ = s_.rest_density * powf(2.0f * r, 3.0f) / 1.0f;
-    poly6_eval_ = (float)(315.0 / (64.0 * M_PI * std::pow((double)h, 9)));
-    poly6_grad_ = (float)(-945.0 / (32.0 * M_PI * std::pow((double)h, 9)));
-    spiky_grad_ = (float)(-45.0 / (M_PI * std::pow((double)h, 6)));
-    visc_lapl_ = (float)(45.0 / (M_PI * std::pow((double)h, 6)));
-    st_main_ = (float)(32.0f / (M_PI * (double)powf(h, 9.0f)));
-    st_term_ = powf(h, 6.0f) / 64.0f;
+}
+
+// _initialize_params (wcsphsimluation.cpp:165-202, 188-199) and the kernel
+// arguments of simulate (wcsphsimluation.cpp:216-282), shared by the
+// single-scene solver and the batched one so both pass the same bits.
+WcStepParams wcsph_step_params(const Settings& s, const float container[3]) {
+    WcStepParams o{};
+    const float r = s.fluid_particle_radius, h = s.fluid_support_radius;
+    const float mass = s.rest_density * powf(2.0f * r, 3.0f) / 1.0f;
+    const float poly6_eval = (float)(315.0 / (64.0 * M_PI * std::pow((double)h, 9)));
+    const float poly6_grad = (float)(-945.0 / (32.0 * M_PI * std::pow((double)h, 9)));
+    const float spiky_grad = (float)(-45.0 / (M_PI * std::pow((double)h, 6)));
+    const float visc_lapl = (float)(45.0 / (M_PI * std::pow((double)h, 6)));
+    const float st_main = (float)(32.0f / (M_PI * (double)powf(h, 9.0f)));
+    const float st_term = powf(h, 6.0f) / 64.0f;
+    o.D.h = h;
+    o.D.M = mass;
+    o.D.w_eval = poly6_eval;
+    o.D.rho0 = s.rest_density;
+    o.D.k_gas = s.gas_stiffness;
+    o.D.fix_density = (s.flags & SPH_FIX_WCSPH_DENSITY) ? 1 : 0;
+    // compute_normals with the Q8 repair: PARTICLE_MASS is the WCSPH mass
+    // printed like every other -D float constant
+    const float H = define_literal(h);
+    o.nH2 = H * H;
+    o.nw_grad = poly6_grad;
+    o.nM = define_literal(mass);
+    o.nH = H;
+    WcParams& P = o.P;
+    P.h = h;
+    P.M = mass;
+    P.spiky_grad = spiky_grad;
+    P.visc_lapl = visc_lapl;
+    P.k_visc = s.k_viscosity;
+    P.sigma = s.surface_tension;
+    P.rho0 = s.rest_density;
+    P.st_main = st_main;
+    P.st_term = st_term;
+    P.dt = s.time_step;
+    P.max_vel = s.max_vel;
+    P.gx = 0.0f;
+    P.gy = -s.gravity;
+    P.gz = 0.0f;
+    P.lo_x = -container[0] / 2.0f;
+    P.hi_x = container[0] / 2.0f;
+    P.lo_y = -container[1] / 2.0f;
+    P.lo_z = -container[2] / 2.0f;
+    P.hi_z = container[2] / 2.0f;
+    return o;
 }
 
 // WCSPHSimulation::simulate (wcsphsimluation.cpp:40-106)
@@ -913,47 +956,18 @@ void WCSPHSimulation::simulate() {
     if (n_ == 0) return;
     neighbour_search();
     const int nb = boundary_.count();
-    const float h = s_.fluid_support_radius;
-    DensityParams D{};
-    D.h = h;
-    D.M = mass_;
-    D.w_eval = poly6_eval_;
-    D.rho0 = s_.rest_density;
-    D.k_gas = s_.gas_stiffness;
-    D.fix_density = (s_.flags & SPH_FIX_WCSPH_DENSITY) ? 1 : 0;
+    const WcStepParams W = wcsph_step_params(s_, container);
     launch_lists_density(true, n_, d_pos_s.get(), d_ranges.get(), nb, boundary_.d_pos_s.get(),
-                         boundary_.d_phi_s.get(), boundary_.d_ranges.get(), boundary_.d_near.get(), grid_.dev, D,
+                         boundary_.d_phi_s.get(), boundary_.d_ranges.get(), boundary_.d_near.get(), grid_.dev, W.D,
                          d_nl.get(), d_nc.get(), d_bnl.get(), d_bnc.get(), d_density.get(), d_pressure.get(),
                          d_prho.get(), d_state.get(), s, &prof_);
-    // compute_normals with the Q8 repair: PARTICLE_MASS is the WCSPH mass
-    // printed like every other -D float constant
-    float H = define_literal(h);
-    launch_normals(n_, d_prho.get(), d_nl.get(), d_nc.get(), H * H, poly6_grad_, define_literal(mass_), H,
-                   d_normal.get(), s, &prof_);
+    launch_normals(n_, d_prho.get(), d_nl.get(), d_nc.get(), W.nH2, W.nw_grad, W.nM, W.nH, d_normal.get(), s,
+                   &prof_);
     boundary_.apply_fluid_forces();
-    WcParams P{};
-    P.h = h;
-    P.M = mass_;
-    P.spiky_grad = spiky_grad_;
-    P.visc_lapl = visc_lapl_;
-    P.k_visc = s_.k_viscosity;
-    P.sigma = s_.surface_tension;
-    P.rho0 = s_.rest_density;
-    P.st_main = st_main_;
-    P.st_term = st_term_;
-    P.dt = s_.time_step;
-    P.max_vel = s_.max_vel;
-    P.gx = 0.0f;
-    P.gy = -s_.gravity;
-    P.gz = 0.0f;
-    P.lo_x = -container[0] / 2.0f;
-    P.hi_x = container[0] / 2.0f;
-    P.lo_y = -container[1] / 2.0f;
-    P.lo_z = -container[2] / 2.0f;
-    P.hi_z = container[2] / 2.0f;
     launch_accel_integrate(n_, d_prho.get(), d_vel_s.get(), d_vh_s.get(), d_pressure.get(), d_normal.get(),
                            d_nl.get(), d_nc.get(), d_bnl.get(), d_bnc.get(), boundary_.d_pos_s.get(),
-                           boundary_.d_phi_s.get(), P, d_accel.get(), d_pos.get(), d_vel.get(), d_vh.get(), s, &prof_);
+                           boundary_.d_phi_s.get(), W.P, d_accel.get(), d_pos.get(), d_vel.get(), d_vh.get(), s,
+                           &prof_);
 }
 
 std::unique_ptr<SimulationBase> build_simulation(const Settings& s) {

@@ -268,8 +268,15 @@ public:
 protected:
     void initialize_params() override;
     float mass_ = 0.0f;
-    float poly6_eval_ = 0, poly6_grad_ = 0, spiky_grad_ = 0, visc_lapl_ = 0, st_main_ = 0, st_term_ = 0;
 };
+
+// Kernel arguments of one WCSPH step (density, normals, acceleration).
+struct WcStepParams {
+    DensityParams D;
+    WcParams P;
+    float nH2, nw_grad, nM, nH;
+};
+WcStepParams wcsph_step_params(const Settings& s, const float container[3]);
 
 // FluidSimulationFactory::build_simulation (fluidsimulationfactory.h:13-40)
 std::unique_ptr<SimulationBase> build_simulation(const Settings& s);

@@ -5,6 +5,7 @@
 #include <cstring>
 #include <string>
 
+#include "batch_host.h"
 #include "rigid.h"
 #include "solver.h"
 #include "sph.h"
@@ -13,6 +14,10 @@ using namespace sph;
 
 struct sph_handle {
     std::unique_ptr<SimulationBase> sim;
+};
+
+struct sph_batch {
+    BatchWCSPH batch;
 };
 
 struct sph_rigid_world {
@@ -481,6 +486,187 @@ int sph_coupled_step(sph_handle* h, sph_rigid_world* w, int npairs, const int* s
         W.step(S.settings().time_step);
         return SPH_OK;
     });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ batched WCSPH
+namespace {
+BatchWCSPH& batch_of(sph_batch* b) {
+    if (!b) throw Error(SPH_ERR_INVALID, "batch is NULL");
+    return b->batch;
+}
+
+int copy_kernel_times(Profiler& p, char* names, int* launches, double* total_ms, int cap) {
+    p.resolve();
+    int n = (int)p.entries.size();
+    for (int k = 0; k < n && k < cap; ++k) {
+        if (names) {
+            std::strncpy(names + 48 * k, p.entries[k].name.c_str(), 47);
+            names[48 * k + 47] = 0;
+        }
+        if (launches) launches[k] = (int)p.entries[k].launches;
+        if (total_ms) total_ms[k] = p.entries[k].total_ms;
+    }
+    return n;
+}
+}  // namespace
+
+extern "C" {
+
+sph_batch* sph_batch_create(void) {
+    sph_batch* out = nullptr;
+    int rc = guard([&] {
+        out = new sph_batch();
+        return SPH_OK;
+    });
+    return rc == SPH_OK ? out : nullptr;
+}
+
+int sph_batch_destroy(sph_batch* b) {
+    return guard([&] {
+        delete b;
+        return SPH_OK;
+    });
+}
+
+int sph_batch_add_scene(sph_batch* b, const sph_settings* s, float width, float height, float depth,
+                        const float* xyzw, int n) {
+    return guard([&] {
+        validate(s);
+        const float c[3] = {width, height, depth};
+        return batch_of(b).add_scene(to_settings(s), c, xyzw, n);
+    });
+}
+
+int sph_batch_finalize(sph_batch* b) {
+    return guard([&] {
+        batch_of(b).finalize();
+        return SPH_OK;
+    });
+}
+
+int sph_batch_autotune(sph_batch* b, const char* cache_path, int trials) {
+    return guard([&] { return batch_of(b).autotune(cache_path, trials); });
+}
+
+int sph_batch_get_launch(sph_batch* b, int kernel, int* block, int* grid, int* stage) {
+    return guard([&] {
+        if (kernel < 0 || kernel >= BatchWCSPH::K_COUNT) throw Error(SPH_ERR_RANGE, "batch: no such kernel");
+        const LaunchCfg& c = batch_of(b).config(kernel);
+        if (block) *block = c.block;
+        if (grid) *grid = c.grid;
+        if (stage) *stage = c.stage;
+        return SPH_OK;
+    });
+}
+
+int sph_batch_set_launch(sph_batch* b, int kernel, int block, int grid, int stage) {
+    return guard([&] {
+        LaunchCfg c;
+        c.block = block;
+        c.grid = grid;
+        c.stage = stage;
+        batch_of(b).set_config(kernel, c);
+        return SPH_OK;
+    });
+}
+
+int sph_batch_set_graph(sph_batch* b, int on) {
+    return guard([&] {
+        batch_of(b).set_graph(on != 0);
+        return SPH_OK;
+    });
+}
+
+int sph_batch_step(sph_batch* b, int nsteps) {
+    return guard([&] {
+        batch_of(b).step(nsteps);
+        return SPH_OK;
+    });
+}
+
+int sph_batch_synchronize(sph_batch* b) {
+    return guard([&] {
+        batch_of(b).synchronize();
+        return SPH_OK;
+    });
+}
+
+int sph_batch_scene_count(sph_batch* b) {
+    return guard([&] { return batch_of(b).scene_count(); });
+}
+
+int sph_batch_particle_count(sph_batch* b, int scene) {
+    return guard([&] { return batch_of(b).scene_particles(scene); });
+}
+
+int sph_batch_total_particles(sph_batch* b) {
+    return guard([&] { return batch_of(b).total_particles(); });
+}
+
+int sph_batch_get_positions(sph_batch* b, int scene, float* xyzw, int n) {
+    return guard([&] {
+        batch_of(b).copy_scene(0, scene, xyzw, n);
+        return SPH_OK;
+    });
+}
+
+int sph_batch_get_velocities(sph_batch* b, int scene, float* xyzw, int n) {
+    return guard([&] {
+        batch_of(b).copy_scene(1, scene, xyzw, n);
+        return SPH_OK;
+    });
+}
+
+int sph_batch_get_half_velocities(sph_batch* b, int scene, float* xyzw, int n) {
+    return guard([&] {
+        batch_of(b).copy_scene(2, scene, xyzw, n);
+        return SPH_OK;
+    });
+}
+
+int sph_batch_set_positions(sph_batch* b, int scene, const float* xyzw, int n) {
+    return guard([&] {
+        batch_of(b).set_scene_state(0, scene, xyzw, n);
+        return SPH_OK;
+    });
+}
+
+int sph_batch_set_velocities(sph_batch* b, int scene, const float* xyzw, int n) {
+    return guard([&] {
+        batch_of(b).set_scene_state(1, scene, xyzw, n);
+        return SPH_OK;
+    });
+}
+
+int sph_batch_get_density(sph_batch* b, int scene, float* out, int n) {
+    return guard([&] {
+        batch_of(b).copy_scene(3, scene, out, n);
+        return SPH_OK;
+    });
+}
+
+int sph_batch_get_neighbour_counts(sph_batch* b, int scene, int* out, int n) {
+    return guard([&] {
+        batch_of(b).copy_scene(4, scene, reinterpret_cast<float*>(out), n);
+        return SPH_OK;
+    });
+}
+
+int sph_batch_set_profiling(sph_batch* b, int on, const char* kernel) {
+    return guard([&] {
+        Profiler& p = batch_of(b).profiler();
+        p.resolve();
+        p.reset();
+        p.filter = kernel ? kernel : "";
+        p.enabled = on != 0;
+        return SPH_OK;
+    });
+}
+
+int sph_batch_get_kernel_times(sph_batch* b, char* names, int* launches, double* total_ms, int cap) {
+    return guard([&] { return copy_kernel_times(batch_of(b).profiler(), names, launches, total_ms, cap); });
 }
 
 }  // extern "C"

@@ -4,6 +4,7 @@
 // the acceleration, stores it and advances the particle. Density, Tait
 // pressure and the lists come from k_lists_density<true> (grid.hip), the
 // normals from k_normals.
+#include "batch.h"
 #include "kernels.h"
 #include "sph_device.h"
 
@@ -24,14 +25,15 @@ __device__ __forceinline__ float st_kernel_wc(float r, float h, float c) {
 
 constexpr int WC_BATCH = 4;
 
-__global__ __launch_bounds__(256) void k_accel_integrate(
-    int n, const float4* __restrict__ prho, const float4* __restrict__ vel_s, const float4* __restrict__ vh_s,
+// One particle of compute_acceleration + predict_vel_n_pos; n is the list
+// stride (the total particle count).
+__device__ __forceinline__ void accel_integrate_one(
+    int i, int n, const float4* __restrict__ prho, const float4* __restrict__ vel_s, const float4* __restrict__ vh_s,
     const float* __restrict__ pressure, const float4* __restrict__ normal, const int* __restrict__ nl,
     const int* __restrict__ nc, const int* __restrict__ bnl, const int* __restrict__ bnc,
-    const float4* __restrict__ bpos_s, const float* __restrict__ bphi_s, WcParams P, float4* __restrict__ accel,
-    float4* __restrict__ pos_out, float4* __restrict__ vel_out, float4* __restrict__ vh_out) {
-    int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const float4* __restrict__ bpos_s, const float* __restrict__ bphi_s, const WcParams& P,
+    float4* __restrict__ accel, float4* __restrict__ pos_out, float4* __restrict__ vel_out,
+    float4* __restrict__ vh_out) {
     float4 p4 = prho[i];
     V3 pi = v3(p4), vi = v3(vel_s[i]);
     float di = p4.w, pri = pressure[i];
@@ -115,6 +117,18 @@ __global__ __launch_bounds__(256) void k_accel_integrate(
     vel_out[i] = f4(clamp3(vt, -P.max_vel, P.max_vel), 0.0f);
 }
 
+__global__ __launch_bounds__(256) void k_accel_integrate(
+    int n, const float4* __restrict__ prho, const float4* __restrict__ vel_s, const float4* __restrict__ vh_s,
+    const float* __restrict__ pressure, const float4* __restrict__ normal, const int* __restrict__ nl,
+    const int* __restrict__ nc, const int* __restrict__ bnl, const int* __restrict__ bnc,
+    const float4* __restrict__ bpos_s, const float* __restrict__ bphi_s, WcParams P, float4* __restrict__ accel,
+    float4* __restrict__ pos_out, float4* __restrict__ vel_out, float4* __restrict__ vh_out) {
+    int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    accel_integrate_one(i, n, prho, vel_s, vh_s, pressure, normal, nl, nc, bnl, bnc, bpos_s, bphi_s, P, accel,
+                        pos_out, vel_out, vh_out);
+}
+
 void launch_accel_integrate(int n, const float4* prho, const float4* vel_s, const float4* vh_s,
                             const float* pressure, const float4* normal, const int* nl, const int* nc,
                             const int* bnl, const int* bnc, const float4* bpos_s, const float* bphi_s,
@@ -124,6 +138,36 @@ void launch_accel_integrate(int n, const float4* prho, const float4* vel_s, cons
     PROF_BEGIN(prof, "accel_integrate", s);
     hipLaunchKernelGGL(k_accel_integrate, dim3((n + 255) / 256), dim3(256), 0, s, n, prho, vel_s, vh_s, pressure,
                        normal, nl, nc, bnl, bnc, bpos_s, bphi_s, P, accel, pos_out, vel_out, vh_out);
+    PROF_END(prof, s);
+}
+
+
+// One step's acceleration and integration over a packed batch of scenes
+// (batch.h); no scene has boundary particles.
+__global__ __launch_bounds__(256) void k_batch_accel_integrate(
+    int n, const int* __restrict__ sid, const BatchScene* __restrict__ sc, const float4* __restrict__ prho,
+    const float4* __restrict__ vel_s, const float4* __restrict__ vh_s, const float* __restrict__ pressure,
+    const float4* __restrict__ normal, const int* __restrict__ nl, const int* __restrict__ nc,
+    const int* __restrict__ bnc, float4* __restrict__ accel, float4* __restrict__ pos_out,
+    float4* __restrict__ vel_out, float4* __restrict__ vh_out) {
+    for (int base = xcd_block(blockIdx.x, gridDim.x) * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        int i = base + threadIdx.x;
+        if (i >= n) break;
+        const BatchScene& S = sc[sid[i]];
+        accel_integrate_one(i, n, prho, vel_s, vh_s, pressure, normal, nl, nc, nullptr, bnc, nullptr, nullptr, S.P,
+                            accel, pos_out, vel_out, vh_out);
+    }
+}
+
+void launch_batch_accel_integrate(int n, const int* sid, const BatchScene* sc, const float4* prho,
+                                  const float4* vel_s, const float4* vh_s, const float* pressure,
+                                  const float4* normal, const int* nl, const int* nc, const int* bnc, float4* accel,
+                                  float4* pos_out, float4* vel_out, float4* vh_out, const LaunchCfg& c,
+                                  hipStream_t s, Profiler* prof) {
+    if (n <= 0) return;
+    PROF_BEGIN(prof, "batch_accel_integrate", s);
+    hipLaunchKernelGGL(k_batch_accel_integrate, dim3(batch_blocks(n, c)), dim3(c.block), 0, s, n, sid, sc, prho,
+                       vel_s, vh_s, pressure, normal, nl, nc, bnc, accel, pos_out, vel_out, vh_out);
     PROF_END(prof, s);
 }
 

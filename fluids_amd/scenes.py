@@ -261,3 +261,60 @@ def build(sc: Scene, sim_cls=None):
     for b in sc.bodies:
         sim.add_boundary(b.samples, b.can_move, b.mass, b.R, b.com)
     return sim
+
+
+def c4_batched(seed: int = 404, count: int = 512, n_min: int = 1000, n_max: int = 64000) -> List[Scene]:
+    """Batched WCSPH scenes, drawn in order from one SplitMix64 stream:
+    particle count N = round(exp(U(ln n_min, ln n_max))), tank sides
+    U(0.5, 2) m each, Monaghan alpha U(0.005, 0.05) and c0 U(10, 40) m/s
+    (mapped as in c1_wcsph_dambreak), spacing s U(0.01, 0.02) m clamped so
+    that N s^3 fits in 40 % of the tank, h = 2 s, and dt = 0.25 h / c0 (the
+    reference has no CFL rule; this is the stated one). The fluid is a dam
+    block of shape 0.5W x 0.8H x D scaled to N lattice points, one spacing
+    off the -x wall and the floor, centred in z; the tank is the integrator's
+    clamp box (no boundary particles). Each scene's count is the block's
+    lattice count, close to N."""
+    rng = Rng(seed)
+    out = []
+    rho0 = 1000.0
+    for k in range(count):
+        u = rng.uniform(7)
+        n_target = int(round(np.exp(np.log(n_min) + u[0] * (np.log(n_max) - np.log(n_min)))))
+        W, H, D = (0.5 + 1.5 * u[1:4]).tolist()
+        alpha = 0.005 + 0.045 * u[4]
+        c0 = 10.0 + 30.0 * u[5]
+        s = 0.01 + 0.01 * u[6]
+        s = min(s, (0.4 * W * H * D / n_target) ** (1.0 / 3.0))
+        kk = (n_target * s ** 3 / (0.4 * W * H * D)) ** (1.0 / 3.0)
+        nx = max(1, int(round(kk * 0.5 * W / s)))
+        ny = max(1, int(round(kk * 0.8 * H / s)))
+        nz = max(1, int(round(n_target / (nx * ny))))
+        nz = max(1, min(nz, int((D - 2 * s) / s)))
+        size = np.array([nx, ny, nz], np.float64) * s * (1 - 1e-5)
+        center = [-W / 2 + s + size[0] / 2, -H / 2 + s + size[1] / 2, 0.0]
+        h, r = 2 * s, s / 2
+        m = rho0 * s ** 3
+        nu = alpha * (h / 2) * c0 / 10
+        settings = dict(time_step=0.25 * h / c0, max_vel=10.0, fluid_particle_radius=r, fluid_support_radius=h,
+                        pcisph_max_iterations=7, pcisph_error_ratio=0.01, method=WCSPH, rest_density=rho0,
+                        k_viscosity=nu * m, gravity=9.81, gas_stiffness=rho0 * c0 * c0 / 7.0,
+                        surface_tension=0.001, flags=FIX_WCSPH_DENSITY)
+        vol = _box(size.astype(np.float32), np.array(center, np.float32), r, rng)
+        assert vol.shape[0] == nx * ny * nz, (vol.shape, nx, ny, nz)
+        out.append(Scene(f"batched_wcsph_{k}", settings, (W, H, D), [vol], [], steps=1000,
+                         notes=f"alpha={alpha:.5f} c0={c0:.3f} target_n={n_target}"))
+    return out
+
+
+def build_batch(scs: List[Scene], autotune: bool = True, cache: Optional[str] = None, trials: int = 2):
+    """The scenes as one libsph batch (sph_batch_*), finalised and, unless
+    told otherwise, autotuned."""
+    from .batch import BatchSimulation
+    b = BatchSimulation()
+    for sc in scs:
+        parts = np.concatenate(sc.volumes) if sc.volumes else np.zeros((0, 4), np.float32)
+        b.add_scene(sc.settings, sc.container, parts)
+    b.finalize()
+    if autotune:
+        b.autotune(cache, trials)
+    return b

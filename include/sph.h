@@ -298,6 +298,55 @@ int sph_rigid_body_count(sph_rigid_world* w);
  * steps by the fluid's time step. Synchronises the fluid's stream. */
 int sph_coupled_step(sph_handle* h, sph_rigid_world* w, int npairs, const int* sph_ids, const int* rigid_ids);
 
+/* ------------------------------------------------------ batched WCSPH (C4)
+ * Many independent WCSPH scenes advanced together: the scenes' particles are
+ * packed into one array and every step is one set of launches over all of
+ * them (captured once into a HIP graph and replayed), with per-scene grids
+ * and parameters. A scene in a batch produces the same bits as the same
+ * scene run alone through sph_create / sph_step. No reference counterpart:
+ * the reference runs one scene per process (fluidsimulationfactory.h:13-40);
+ * callers are sweeps and ensembles of small scenes. Scenes have no boundary
+ * particles (their tanks are the integrator's clamp box, Q15). */
+typedef struct sph_batch sph_batch;
+
+sph_batch* sph_batch_create(void);
+int sph_batch_destroy(sph_batch* b);
+/* A scene: WCSPH settings, its container (as sph_set_rect_limits) and its
+ * particles (xyzw, as sph_add_volume). Returns the scene id. */
+int sph_batch_add_scene(sph_batch* b, const sph_settings* s, float width, float height, float depth,
+                        const float* xyzw, int n);
+/* Packs the scenes and allocates device memory; no scene can be added after. */
+int sph_batch_finalize(sph_batch* b);
+/* Times the launch shapes of each batched kernel (threads per block, blocks,
+ * LDS-staged list slots) on this batch and keeps the fastest; the scenes'
+ * state is restored afterwards. cache_path (may be NULL) names a text file of
+ * earlier results for the same device and batch size. Returns 1 if the cache
+ * supplied the shapes, 0 if they were measured. */
+int sph_batch_autotune(sph_batch* b, const char* cache_path, int trials);
+/* kernel: 0 hash, 1 gather+ranges, 2 lists+density, 3 normals, 4 accel+integrate */
+int sph_batch_get_launch(sph_batch* b, int kernel, int* block, int* grid, int* stage);
+int sph_batch_set_launch(sph_batch* b, int kernel, int block, int grid, int stage);
+/* Graph replay of the step (on by default). */
+int sph_batch_set_graph(sph_batch* b, int on);
+int sph_batch_step(sph_batch* b, int nsteps);
+int sph_batch_synchronize(sph_batch* b);
+int sph_batch_scene_count(sph_batch* b);
+int sph_batch_particle_count(sph_batch* b, int scene);
+int sph_batch_total_particles(sph_batch* b);
+/* Stored-order float4 state of one scene (synchronises). */
+int sph_batch_get_positions(sph_batch* b, int scene, float* xyzw, int n);
+int sph_batch_get_velocities(sph_batch* b, int scene, float* xyzw, int n);
+int sph_batch_get_half_velocities(sph_batch* b, int scene, float* xyzw, int n);
+int sph_batch_set_positions(sph_batch* b, int scene, const float* xyzw, int n);
+int sph_batch_set_velocities(sph_batch* b, int scene, const float* xyzw, int n);
+/* Last step's densities (float) and neighbour counts (int) of one scene, in
+ * its sorted order. */
+int sph_batch_get_density(sph_batch* b, int scene, float* out, int n);
+int sph_batch_get_neighbour_counts(sph_batch* b, int scene, int* out, int n);
+/* HIP-event timing of the batched launches, as sph_set_profiling. */
+int sph_batch_set_profiling(sph_batch* b, int on, const char* kernel);
+int sph_batch_get_kernel_times(sph_batch* b, char* names, int* launches, double* total_ms, int cap);
+
 #ifdef __cplusplus
 }
 #endif
