@@ -111,6 +111,48 @@ def run_batched(steps, warmup, plain_steps, count=512):
             "speedup_vs_plain": v / vp, "launch_shapes": shapes, "build_and_tune_s": t_tune}
 
 
+def run_adjoint(repeats=3):
+    """WCSPH adjoint on the 16 384-particle inverse problem
+    (fluids_amd.scenes.c5_adjoint: 200 steps, checkpoint every 20): wall time
+    of the plain solver's 200 steps, of the adjoint's forward pass (same
+    steps, plus tapes and checkpoints) and of its backward pass (recomputing
+    each segment from its checkpoint, then the adjoint kernels), best of
+    `repeats`."""
+    import numpy as np
+    from fluids_amd import scenes
+    from fluids_amd.adjoint import WCSPHAdjoint
+    case = scenes.c5_adjoint()
+    sc = case.scene
+    n = sc.n_fluid
+    v4 = np.concatenate([case.v0_hidden, np.zeros((n, 1), np.float32)], 1)
+    plain = []
+    for _ in range(repeats):
+        sim = scenes.build(sc)
+        sim.set_velocities(v4)
+        sim.set_half_velocities(v4)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.step(case.steps)
+        sim.synchronize()
+        plain.append(time.perf_counter() - t0)
+        sim.close()
+    adj = WCSPHAdjoint(sc.settings, sc.container, sc.volumes[0], v4, v4, checkpoint_every=case.checkpoint_every)
+    fwd, bwd = [], []
+    g = np.ones((n, 3), np.float32) / n
+    for _ in range(repeats):
+        adj.forward(case.steps)
+        adj.backward(g)
+        f, b = adj.timings()
+        fwd.append(f / 1e3)
+        bwd.append(b / 1e3)
+    adj.close()
+    tp, tf, tb = min(plain), min(fwd), min(bwd)
+    return {"workload": "wcsph_adjoint_16k", "particles": n, "steps": case.steps,
+            "checkpoint_every": case.checkpoint_every, "plain_forward_ms": 1e3 * tp,
+            "adjoint_forward_ms": 1e3 * tf, "backward_ms": 1e3 * tb,
+            "backward_over_plain_forward": tb / tp, "forward_plus_backward_over_plain_forward": (tf + tb) / tp}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +168,7 @@ def main():
     ap.add_argument("--batched-only", action="store_true", help="only the batched-scenes measurement")
     ap.add_argument("--batched-steps", type=int, default=50)
     ap.add_argument("--plain-steps", type=int, default=5)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "pmc_pcisph_throughput_1000k.md"),
                     help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary (python -m fluids_amd.profiling pmc) "
                          "that gives the dominant kernel's HBM bytes per launch")
@@ -140,7 +183,8 @@ def main():
     from fluids_amd._lib import lib
     lib()
     if a.batched_only:
-        print(json.dumps({"batched": run_batched(a.batched_steps, 5, a.plain_steps)}), flush=True)
+        print(json.dumps({"batched": run_batched(a.batched_steps, 5, a.plain_steps),
+                          "adjoint": None if a.no_adjoint else run_adjoint()}), flush=True)
         return
     t_init = time.perf_counter()
     sc = scenes.c3_pcisph_throughput(a.n)
@@ -214,6 +258,7 @@ def main():
     batched = None
     if not a.no_batched:
         batched = run_batched(a.batched_steps, 5, a.plain_steps)
+    adjoint = None if a.no_adjoint else run_adjoint()
 
     rec = {
         "metric": "particle-updates/s (PCISPH 1M fp32), %HBM roofline; batched & adjoint vs plain",
@@ -227,6 +272,7 @@ def main():
         "step_roofline": {"alg_bytes_per_pu": bpp, "achieved_GBps": bpp * pu_s / 1e9, "frac": bpp * pu_s / HBM_PEAK},
         "cpu_baseline": cpu,
         "batched": batched,
+        "adjoint": adjoint,
         "kernels": per,
         "events_in_timed_region": not a.no_events,
     }

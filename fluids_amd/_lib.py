@@ -125,6 +125,14 @@ SIGNATURES = {
     "sph_batch_get_neighbour_counts": (I, [P, I, P, I]),
     "sph_batch_set_profiling": (I, [P, I, C.c_char_p]),
     "sph_batch_get_kernel_times": (I, [P, P, P, P, I]),
+    "sph_adjoint_create": (P, [C.POINTER(SphSettings), F, F, F, P, P, P, I, I]),
+    "sph_adjoint_destroy": (I, [P]),
+    "sph_adjoint_set_initial": (I, [P, P, P, P]),
+    "sph_adjoint_set_params": (I, [P, F, F, F]),
+    "sph_adjoint_forward": (I, [P, I]),
+    "sph_adjoint_final_state": (I, [P, P, P, P]),
+    "sph_adjoint_backward": (I, [P, P, P, P, P, P, P, P]),
+    "sph_adjoint_timings": (I, [P, P, P]),
 }
 
 _LIB = None

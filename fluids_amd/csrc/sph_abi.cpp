@@ -5,6 +5,7 @@
 #include <cstring>
 #include <string>
 
+#include "adjoint_host.h"
 #include "batch_host.h"
 #include "rigid.h"
 #include "solver.h"
@@ -14,6 +15,10 @@ using namespace sph;
 
 struct sph_handle {
     std::unique_ptr<SimulationBase> sim;
+};
+
+struct sph_adjoint {
+    std::unique_ptr<WcAdjoint> adj;
 };
 
 struct sph_batch {
@@ -667,6 +672,82 @@ int sph_batch_set_profiling(sph_batch* b, int on, const char* kernel) {
 
 int sph_batch_get_kernel_times(sph_batch* b, char* names, int* launches, double* total_ms, int cap) {
     return guard([&] { return copy_kernel_times(batch_of(b).profiler(), names, launches, total_ms, cap); });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------- adjoint
+namespace {
+WcAdjoint& adjoint_of(sph_adjoint* a) {
+    if (!a || !a->adj) throw Error(SPH_ERR_INVALID, "adjoint is NULL");
+    return *a->adj;
+}
+}  // namespace
+
+extern "C" {
+
+sph_adjoint* sph_adjoint_create(const sph_settings* s, float width, float height, float depth, const float* pos,
+                                const float* vel, const float* vel_half, int n, int checkpoint_every) {
+    sph_adjoint* out = nullptr;
+    int rc = guard([&] {
+        validate(s);
+        const float c[3] = {width, height, depth};
+        std::unique_ptr<WcAdjoint> a(new WcAdjoint(to_settings(s), c, pos, vel, vel_half, n, checkpoint_every));
+        out = new sph_adjoint{std::move(a)};
+        return SPH_OK;
+    });
+    return rc == SPH_OK ? out : nullptr;
+}
+
+int sph_adjoint_destroy(sph_adjoint* a) {
+    return guard([&] {
+        delete a;
+        return SPH_OK;
+    });
+}
+
+int sph_adjoint_set_initial(sph_adjoint* a, const float* pos, const float* vel, const float* vel_half) {
+    return guard([&] {
+        adjoint_of(a).set_initial(pos, vel, vel_half);
+        return SPH_OK;
+    });
+}
+
+int sph_adjoint_set_params(sph_adjoint* a, float k_viscosity, float gas_stiffness, float surface_tension) {
+    return guard([&] {
+        adjoint_of(a).set_params(k_viscosity, gas_stiffness, surface_tension);
+        return SPH_OK;
+    });
+}
+
+int sph_adjoint_forward(sph_adjoint* a, int nsteps) {
+    return guard([&] {
+        adjoint_of(a).forward(nsteps);
+        return SPH_OK;
+    });
+}
+
+int sph_adjoint_final_state(sph_adjoint* a, float* pos, float* vel, float* vel_half) {
+    return guard([&] {
+        adjoint_of(a).final_state(pos, vel, vel_half);
+        return SPH_OK;
+    });
+}
+
+int sph_adjoint_backward(sph_adjoint* a, const float* g_pos, const float* g_vel, const float* g_vel_half,
+                         float* g_pos0, float* g_vel0, float* g_vel_half0, double* g_params) {
+    return guard([&] {
+        adjoint_of(a).backward(g_pos, g_vel, g_vel_half, g_pos0, g_vel0, g_vel_half0, g_params);
+        return SPH_OK;
+    });
+}
+
+int sph_adjoint_timings(sph_adjoint* a, double* forward_ms, double* backward_ms) {
+    return guard([&] {
+        if (forward_ms) *forward_ms = adjoint_of(a).forward_ms();
+        if (backward_ms) *backward_ms = adjoint_of(a).backward_ms();
+        return SPH_OK;
+    });
 }
 
 }  // extern "C"

@@ -318,3 +318,62 @@ def build_batch(scs: List[Scene], autotune: bool = True, cache: Optional[str] = 
     if autotune:
         b.autotune(cache, trials)
     return b
+
+
+def monaghan_k_viscosity(alpha, c0, h, mass):
+    """Monaghan alpha mapped onto the reference's Laplacian viscosity (as in
+    c1_wcsph_dambreak): k_viscosity = nu * m, nu = alpha (h/2) c0 / 10.
+    Works on floats and on torch tensors."""
+    return alpha * (h / 2) * c0 / 10 * mass
+
+
+def tait_gas_stiffness(c0, rho0):
+    """Tait EOS with gamma = 7: gas_stiffness = rho0 c0^2 / 7."""
+    return rho0 * c0 * c0 / 7.0
+
+
+@dataclass
+class AdjointCase:
+    scene: Scene
+    v0_hidden: np.ndarray  # (n, 3) float32 initial velocities of the target run
+    alpha: float
+    c0: float
+    steps: int
+    checkpoint_every: int
+
+    @property
+    def mass(self) -> float:
+        return self.scene.settings["rest_density"] * (2 * self.scene.settings["fluid_particle_radius"]) ** 3
+
+
+def c5_adjoint(seed: int = 505, nx: int = 32, ny: int = 32, nz: int = 16, steps: int = 200) -> AdjointCase:
+    """Differentiable WCSPH inverse problem: an nx x ny x nz block on a 0.01 m
+    lattice (h = 0.02 m) centred at the origin of a 1 x 1 x 1 m clamp-box
+    container, dt = 2e-4 s. Hidden alpha = 0.03, c0 = 20 m/s. The hidden
+    initial velocity is, per component, a sum of 8 seeded Fourier modes
+    sin(2 pi k.x / L + phase) with integer k in [-2, 2]^3, amplitude U(-1, 1)
+    and L the block's largest side, scaled to max |v| = 0.5 m/s. Surface
+    tension 0.001 and max_vel 10 come from config.wc; the density is the
+    repaired one."""
+    s, rho0, alpha, c0 = 0.01, 1000.0, 0.03, 20.0
+    r, h = s / 2, 2 * s
+    m = rho0 * s ** 3
+    rng = Rng(seed)
+    size = np.array([nx, ny, nz], np.float64) * s * (1 - 1e-5)
+    vol = _box(size.astype(np.float32), np.zeros(3, np.float32), r, rng)
+    settings = dict(time_step=2e-4, max_vel=10.0, fluid_particle_radius=r, fluid_support_radius=h,
+                    pcisph_max_iterations=7, pcisph_error_ratio=0.01, method=WCSPH, rest_density=rho0,
+                    k_viscosity=monaghan_k_viscosity(alpha, c0, h, m), gravity=9.81,
+                    gas_stiffness=tait_gas_stiffness(c0, rho0), surface_tension=0.001, flags=FIX_WCSPH_DENSITY)
+    L = float(max(nx, ny, nz) * s)
+    x = vol[:, :3].astype(np.float64)
+    v = np.zeros((vol.shape[0], 3))
+    for c in range(3):
+        for _ in range(8):
+            k = (rng.u64() % 125)
+            kv = np.array([k % 5, (k // 5) % 5, k // 25], np.float64) - 2.0
+            a, ph = rng.uniform(2)
+            v[:, c] += (2 * a - 1) * np.sin(2 * np.pi * (x @ kv) / L + 2 * np.pi * ph)
+    v *= 0.5 / np.max(np.linalg.norm(v, axis=1))
+    sc = Scene("wcsph_adjoint_16k", settings, (1.0, 1.0, 1.0), [vol], [], steps=steps)
+    return AdjointCase(sc, v.astype(np.float32), alpha, c0, steps, 20)

@@ -347,6 +347,37 @@ int sph_batch_get_neighbour_counts(sph_batch* b, int scene, int* out, int n);
 int sph_batch_set_profiling(sph_batch* b, int on, const char* kernel);
 int sph_batch_get_kernel_times(sph_batch* b, char* names, int* launches, double* total_ms, int cap);
 
+/* --------------------------------------------------- WCSPH adjoint (C5)
+ * Gradients of a loss on the state after T WCSPH steps with respect to the
+ * initial state and to k_viscosity, gas_stiffness and surface_tension,
+ * computed by a hand-written reverse-mode pass. The forward pass is the
+ * solver's own step (same kernels, same bits as sph_step) with a checkpoint
+ * every C steps; the backward pass recomputes each segment from its
+ * checkpoint and runs the adjoint kernels backwards through it. No reference
+ * counterpart (the reference has no gradients); callers are inverse problems
+ * and parameter fits, e.g. through fluids_amd.adjoint.WCSPHRollout, a
+ * torch.autograd.Function. Scenes have no boundary particles and every
+ * neighbour list must stay below the cap of 50 (else the backward pass
+ * returns SPH_ERR_RANGE). States are float4 per particle in particle-id
+ * order. */
+typedef struct sph_adjoint sph_adjoint;
+
+sph_adjoint* sph_adjoint_create(const sph_settings* s, float width, float height, float depth, const float* pos,
+                                const float* vel, const float* vel_half, int n, int checkpoint_every);
+int sph_adjoint_destroy(sph_adjoint* a);
+int sph_adjoint_set_initial(sph_adjoint* a, const float* pos, const float* vel, const float* vel_half);
+int sph_adjoint_set_params(sph_adjoint* a, float k_viscosity, float gas_stiffness, float surface_tension);
+int sph_adjoint_forward(sph_adjoint* a, int nsteps);
+/* State after the forward pass (any output may be NULL). */
+int sph_adjoint_final_state(sph_adjoint* a, float* pos, float* vel, float* vel_half);
+/* g_* : dL/d(final pos, vel, vel_half) (NULL = 0). Outputs (may be NULL):
+ * dL/d(initial pos, vel, vel_half) and g_params = dL/d(k_viscosity,
+ * gas_stiffness, surface_tension). */
+int sph_adjoint_backward(sph_adjoint* a, const float* g_pos, const float* g_vel, const float* g_vel_half,
+                         float* g_pos0, float* g_vel0, float* g_vel_half0, double* g_params);
+/* Wall-clock milliseconds of the last forward and backward passes. */
+int sph_adjoint_timings(sph_adjoint* a, double* forward_ms, double* backward_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -94,6 +94,9 @@ def lib():
         L.oracle_cube.argtypes = [F, F, P, I]
         L.oracle_wall.restype = I
         L.oracle_wall.argtypes = [F, F, F, P, I]
+        D = C.c_double
+        L.oracle_wcsph_rollout64.restype = I
+        L.oracle_wcsph_rollout64.argtypes = [C.POINTER(Settings), D, D, D, P, P, P, I, I, D, D, D, P, P, P, P]
         _LIB = L
     return _LIB
 
@@ -129,9 +132,12 @@ class Oracle:
             lib().oracle_set_twin(self.h, 1)
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().oracle_destroy(self.h)
-            self.h = None
+        try:
+            if getattr(self, "h", None):
+                lib().oracle_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
 
     def set_rect_limits(self, w, h, d):
         lib().oracle_set_rect_limits(self.h, w, h, d)
@@ -262,3 +268,24 @@ def cube(side, spacing):
 
 def wall(w, h, spacing):
     return _sample(lib().oracle_wall, float(w), float(h), float(spacing))
+
+
+def wcsph_rollout64(settings: dict, container, pos0, vel0=None, vh0=None, steps=1, k_visc=None, k_gas=None,
+                    sigma=None):
+    """WCSPH in double precision with its own neighbour search (for finite
+    differences): returns (pos, vel, vel_half, max list length), (n, 3)
+    float64 arrays in the input's particle order."""
+    st = Settings(**{k: v for k, v in settings.items() if k in dict(Settings._fields_)})
+    x = np.ascontiguousarray(np.asarray(pos0, np.float64)[:, :3])
+    n = x.shape[0]
+    v = None if vel0 is None else np.ascontiguousarray(np.asarray(vel0, np.float64)[:, :3])
+    w = None if vh0 is None else np.ascontiguousarray(np.asarray(vh0, np.float64)[:, :3])
+    po, vo, wo = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+    mc = C.c_int(0)
+    kv = settings["k_viscosity"] if k_visc is None else k_visc
+    kg = settings["gas_stiffness"] if k_gas is None else k_gas
+    sg = settings["surface_tension"] if sigma is None else sigma
+    lib().oracle_wcsph_rollout64(C.byref(st), float(container[0]), float(container[1]), float(container[2]),
+                                 _ptr(x), _ptr(v), _ptr(w), n, int(steps), float(kv), float(kg), float(sg),
+                                 _ptr(po), _ptr(vo), _ptr(wo), C.cast(C.byref(mc), C.c_void_p))
+    return po, vo, wo, mc.value

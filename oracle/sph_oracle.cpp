@@ -1342,3 +1342,191 @@ int oracle_wall(float width, float height, float s, float* out, int cap) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// WCSPH in double precision over double state, with its own neighbour
+// search, for finite differences of a loss on the final state (the gradient
+// checks of the adjoint). It restates wcsph_fields / wcsph_integrate above
+// (wcsph.cl:32-224) with the repaired density when S.flags says so. Lists
+// are rebuilt each step from the double positions with the cutoff
+// r^2 < CELL_SIZE^2 (grid.cl:3) and are uncapped: the checks use scenes whose
+// lists stay below 50. Particles keep their id order; list order is
+// ascending id, which changes nothing but double rounding.
+// ---------------------------------------------------------------------------
+namespace {
+struct D3 {
+    double x, y, z;
+};
+inline D3 operator+(D3 a, D3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline D3 operator-(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline D3 operator*(D3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+inline double ddot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline double dclamp(double x, double lo, double hi) { return std::min(std::max(x, lo), hi); }
+inline double dsign(double x) { return x > 0 ? 1.0 : (x < 0 ? -1.0 : 0.0); }
+inline D3 dclamp3(D3 a, double lo, double hi) { return {dclamp(a.x, lo, hi), dclamp(a.y, lo, hi), dclamp(a.z, lo, hi)}; }
+}  // namespace
+
+extern "C" int oracle_wcsph_rollout64(const oracle_settings* so, double W, double H, double Dp, const double* pos0,
+                                      const double* vel0, const double* vh0, int n, int nsteps, double k_visc,
+                                      double k_gas, double sigma, double* pos_out, double* vel_out, double* vh_out,
+                                      int* max_count) {
+    Settings S;
+    std::memcpy(&S, so, sizeof(Settings));
+    const WcConsts c = wc_consts(S);
+    const double h = c.h, h2 = h * h, M = c.mass, rho0 = S.rest_density, dt = S.time_step, vmax = S.max_vel;
+    const double cut2 = (double)(define_literal(h) * define_literal(h));
+    const double c_rho = (double)c.poly6_eval * M;
+    const double c_n = (double)c.poly6_grad * (double)c.mass_d * (double)c.h_d;
+    const double H2n = (double)c.h_d * (double)c.h_d;
+    const double spiky = c.spiky_grad, visc = (double)c.visc_lapl * k_visc, st_main = c.st_main,
+                 st_term = c.st_term;
+    const bool fix = (S.flags & FIX_WCSPH_DENSITY) != 0;
+    const D3 g = {0.0, -(double)S.gravity, 0.0};
+    std::vector<D3> x(n), v(n), vh(n), nrm(n), acc(n);
+    std::vector<double> rho(n), p(n);
+    for (int i = 0; i < n; ++i) {
+        x[i] = {pos0[3 * i], pos0[3 * i + 1], pos0[3 * i + 2]};
+        v[i] = vel0 ? D3{vel0[3 * i], vel0[3 * i + 1], vel0[3 * i + 2]} : D3{0, 0, 0};
+        vh[i] = vh0 ? D3{vh0[3 * i], vh0[3 * i + 1], vh0[3 * i + 2]} : D3{0, 0, 0};
+    }
+    // neighbour lists in CSR form over a dense cell box rebuilt every step
+    std::vector<int> start(n + 1), cnt(n), nbr;
+    std::vector<int> cell_of(n), cell_start, order(n);
+    int maxc = 0;
+    for (int step = 0; step < nsteps; ++step) {
+        double lo[3] = {1e300, 1e300, 1e300};
+        for (int i = 0; i < n; ++i) {
+            lo[0] = std::min(lo[0], x[i].x);
+            lo[1] = std::min(lo[1], x[i].y);
+            lo[2] = std::min(lo[2], x[i].z);
+        }
+        long long c0[3], nc[3] = {1, 1, 1};
+        for (int k = 0; k < 3; ++k) c0[k] = (long long)std::floor(lo[k] / h) - 1;
+        for (int i = 0; i < n; ++i) {
+            const double q[3] = {x[i].x, x[i].y, x[i].z};
+            for (int k = 0; k < 3; ++k) nc[k] = std::max(nc[k], (long long)std::floor(q[k] / h) - c0[k] + 2);
+        }
+        const long long ncell = nc[0] * nc[1] * nc[2];
+        if (ncell > (1LL << 28)) return -1;
+        auto cidx = [&](const D3& q, int dx, int dy, int dz) {
+            long long cx = (long long)std::floor(q.x / h) - c0[0] + dx;
+            long long cy = (long long)std::floor(q.y / h) - c0[1] + dy;
+            long long cz = (long long)std::floor(q.z / h) - c0[2] + dz;
+            return (cz * nc[1] + cy) * nc[0] + cx;
+        };
+        cell_start.assign(ncell + 1, 0);
+        for (int i = 0; i < n; ++i) {
+            cell_of[i] = (int)cidx(x[i], 0, 0, 0);
+            ++cell_start[cell_of[i] + 1];
+        }
+        for (long long k = 0; k < ncell; ++k) cell_start[k + 1] += cell_start[k];
+        {
+            std::vector<int> fill(cell_start.begin(), cell_start.end() - 1);
+            for (int i = 0; i < n; ++i) order[fill[cell_of[i]]++] = i;
+        }
+        auto walk = [&](int i, auto&& f) {
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        long long k = cidx(x[i], dx, dy, dz);
+                        for (int e = cell_start[k]; e < cell_start[k + 1]; ++e) {
+                            int j = order[e];
+                            D3 r = x[i] - x[j];
+                            if (ddot(r, r) < cut2) f(j);
+                        }
+                    }
+        };
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; ++i) {
+            int c = 0;
+            walk(i, [&](int) { ++c; });
+            cnt[i] = c;
+        }
+        start[0] = 0;
+        for (int i = 0; i < n; ++i) start[i + 1] = start[i] + cnt[i];
+        nbr.resize(start[n]);
+#pragma omp parallel for schedule(static) reduction(max : maxc)
+        for (int i = 0; i < n; ++i) {
+            int o = start[i];
+            walk(i, [&](int j) { nbr[o++] = j; });
+            std::sort(nbr.begin() + start[i], nbr.begin() + start[i + 1]);
+            maxc = std::max(maxc, cnt[i]);
+        }
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; ++i) {
+            double d = 0.0;
+            for (int e = start[i]; e < start[i + 1]; ++e) {
+                const int j = nbr[e];
+                D3 r = x[i] - x[j];
+                double s2 = ddot(r, r);
+                double s = fix ? s2 : std::sqrt(s2);
+                double t = h2 - s;
+                d += t * t * t;
+            }
+            rho[i] = d * c_rho;
+            double y = rho[i] / rho0;
+            p[i] = std::max(0.0, k_gas * (std::pow(y, 7) - 1.0));
+        }
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; ++i) {
+            D3 a = {0, 0, 0};
+            for (int e = start[i]; e < start[i + 1]; ++e) {
+                const int j = nbr[e];
+                D3 r = x[i] - x[j];
+                double w = H2n - ddot(r, r);
+                a = a + r * (w * w / rho[j]);
+            }
+            nrm[i] = a * c_n;
+        }
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; ++i) {
+            D3 pacc = {0, 0, 0}, vacc = {0, 0, 0}, fcurv = {0, 0, 0}, fcoh = {0, 0, 0};
+            const double di = rho[i], Ci = p[i] / (di * di);
+            for (int e = start[i]; e < start[i + 1]; ++e) {
+                const int j = nbr[e];
+                D3 r = x[i] - x[j];
+                double q = std::sqrt(ddot(r, r));
+                double dj = rho[j];
+                if (!(di > 0 && dj > 0 && q > 0.00001)) continue;
+                double hq = h - q;
+                pacc = pacc + r * (spiky / q * hq * hq * (Ci + p[j] / (dj * dj)));
+                vacc = vacc + (v[j] - v[i]) * (hq / dj);
+                double K = 2.0 * rho0 / (di + dj);
+                fcurv = fcurv + (nrm[i] - nrm[j]) * K;
+                double st = 0.0;
+                if (q <= 0.5 * h) st = 2.0 * (hq * hq * hq) * (q * q * q) - st_term;
+                else if (q <= h) st = (hq * hq * hq) * (q * q * q);
+                fcoh = fcoh + r * (st / q * K);
+            }
+            acc[i] = g + (fcurv + fcoh * st_main) * (-sigma) + pacc * (-M) + vacc * visc;
+        }
+        const double lo_x = -W / 2, hi_x = W / 2, lo_y = -H / 2, lo_z = -Dp / 2, hi_z = Dp / 2;
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; ++i) {
+            D3 a = acc[i];
+            D3 w = vh[i] + a * dt;
+            D3 xp = x[i] + dclamp3(w + a * dt, -vmax, vmax) * dt;
+            double vn = std::sqrt(ddot(w, w));
+            D3 cp = {dclamp(xp.x, lo_x, hi_x), dclamp(xp.y, lo_y, 10.0), dclamp(xp.z, lo_z, hi_z)};
+            D3 dv = cp - xp;
+            double dd = std::sqrt(ddot(dv, dv));
+            if (dd > 0 && vn > 0) {
+                D3 sg = {dsign(dv.x), dsign(dv.y), dsign(dv.z)};
+                D3 nn = sg * (1.0 / std::sqrt(ddot(sg, sg)));
+                double rest = dd / (dt * vn);
+                w = w - nn * ((1.0 + rest) * ddot(w, nn));
+                xp = cp;
+            }
+            x[i] = xp;
+            v[i] = dclamp3(w + a * (0.5 * dt), -vmax, vmax);
+            vh[i] = dclamp3(w, -vmax, vmax);
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        if (pos_out) { pos_out[3 * i] = x[i].x; pos_out[3 * i + 1] = x[i].y; pos_out[3 * i + 2] = x[i].z; }
+        if (vel_out) { vel_out[3 * i] = v[i].x; vel_out[3 * i + 1] = v[i].y; vel_out[3 * i + 2] = v[i].z; }
+        if (vh_out) { vh_out[3 * i] = vh[i].x; vh_out[3 * i + 1] = vh[i].y; vh_out[3 * i + 2] = vh[i].z; }
+    }
+    if (max_count) *max_count = maxc;
+    return 0;
+}
