@@ -3,11 +3,13 @@ paivett/fluids, behind the C ABI of include/sph.h (libsph.so)."""
 from ._lib import FIX_WCSPH_DENSITY, PCISPH, WCSPH, SphError
 from .simulation import (FluidSimulation, FluidSimulationFactory, PCISPHSimulation, PhysicsSettings,
                          SimulationSettings, WCSPHSimulation, box_volume_particles, cube_sampling,
-                         fishtank_sampling, load_settings, settings_from_dict, wall_sampling)
+                         fishtank_sampling, load_scene, load_settings, model_sampling, settings_from_dict,
+                         sphere_sampling, wall_sampling)
 
 __all__ = [
     "FluidSimulation", "PCISPHSimulation", "WCSPHSimulation", "FluidSimulationFactory",
     "PhysicsSettings", "SimulationSettings", "load_settings", "settings_from_dict",
-    "box_volume_particles", "fishtank_sampling", "cube_sampling", "wall_sampling",
+    "box_volume_particles", "fishtank_sampling", "cube_sampling", "wall_sampling", "sphere_sampling",
+    "model_sampling", "load_scene",
     "SphError", "WCSPH", "PCISPH", "FIX_WCSPH_DENSITY",
 ]

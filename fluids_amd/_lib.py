@@ -91,6 +91,15 @@ SIGNATURES = {
     "sph_sample_fishtank": (I, [F, F, F, F, P, I]),
     "sph_sample_cube": (I, [F, F, P, I]),
     "sph_sample_wall": (I, [F, F, F, P, I]),
+    "sph_sample_sphere": (I, [F, F, P, I]),
+    "sph_sample_model": (I, [C.c_char_p, F, P, I]),
+    "sph_scene_load": (P, [C.c_char_p, C.POINTER(SphSettings), C.c_char_p]),
+    "sph_scene_destroy": (I, [P]),
+    "sph_scene_simulation": (P, [P]),
+    "sph_scene_world": (P, [P]),
+    "sph_scene_step": (I, [P, I]),
+    "sph_scene_body_count": (I, [P]),
+    "sph_scene_body": (I, [P, I, P, P, P, P, P, P]),
     "sph_rigid_create": (P, [F]),
     "sph_rigid_destroy": (I, [P]),
     "sph_rigid_set_container": (I, [P, F, F, F, P]),

@@ -8,6 +8,7 @@
 #include "adjoint_host.h"
 #include "batch_host.h"
 #include "rigid.h"
+#include "scene.h"
 #include "solver.h"
 #include "sph.h"
 
@@ -26,8 +27,14 @@ struct sph_batch {
 };
 
 struct sph_rigid_world {
-    RigidWorld world;
-    explicit sph_rigid_world(float g) : world(g) {}
+    std::unique_ptr<RigidWorld> own;
+    RigidWorld* world = nullptr;  // own.get(), or a world owned by a scene
+};
+
+struct sph_scene {
+    std::unique_ptr<SceneRun> run;
+    sph_handle handle;
+    sph_rigid_world world;
 };
 
 namespace {
@@ -383,8 +390,8 @@ int sph_sample_wall(float width, float height, float spacing, float* out, int ca
 // ------------------------------------------------------------ rigid bodies
 namespace {
 RigidWorld& world_of(sph_rigid_world* w) {
-    if (!w) throw Error(SPH_ERR_INVALID, "rigid world is NULL");
-    return w->world;
+    if (!w || !w->world) throw Error(SPH_ERR_INVALID, "rigid world is NULL");
+    return *w->world;
 }
 }  // namespace
 
@@ -394,7 +401,9 @@ sph_rigid_world* sph_rigid_create(float gravity_y) {
     sph_rigid_world* out = nullptr;
     int rc = guard([&] {
         if (!std::isfinite(gravity_y)) throw Error(SPH_ERR_INVALID, "gravity must be finite");
-        out = new sph_rigid_world(gravity_y);
+        out = new sph_rigid_world();
+        out->own.reset(new RigidWorld(gravity_y));
+        out->world = out->own.get();
         return SPH_OK;
     });
     return rc == SPH_OK ? out : nullptr;
@@ -402,6 +411,7 @@ sph_rigid_world* sph_rigid_create(float gravity_y) {
 
 int sph_rigid_destroy(sph_rigid_world* w) {
     return guard([&] {
+        if (w && !w->own) throw Error(SPH_ERR_INVALID, "this rigid world belongs to a scene");
         delete w;
         return SPH_OK;
     });
@@ -746,6 +756,108 @@ int sph_adjoint_timings(sph_adjoint* a, double* forward_ms, double* backward_ms)
     return guard([&] {
         if (forward_ms) *forward_ms = adjoint_of(a).forward_ms();
         if (backward_ms) *backward_ms = adjoint_of(a).backward_ms();
+        return SPH_OK;
+    });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ scene files
+namespace sph {
+std::vector<float4> sample_sphere(float radius, float spacing);
+std::vector<float4> sample_model(const std::string& obj_path, float spacing);
+}
+
+namespace {
+sph_scene* scene_of(sph_scene* sc) {
+    if (!sc || !sc->run) throw Error(SPH_ERR_INVALID, "scene is NULL");
+    return sc;
+}
+}  // namespace
+
+extern "C" {
+
+int sph_sample_sphere(float radius, float spacing, float* out, int cap) {
+    return guard([&] { return copy_samples(sample_sphere(radius, spacing), out, cap); });
+}
+
+int sph_sample_model(const char* obj_path, float spacing, float* out, int cap) {
+    return guard([&] {
+        if (!obj_path) throw Error(SPH_ERR_INVALID, "model path is NULL");
+        return copy_samples(sample_model(obj_path, spacing), out, cap);
+    });
+}
+
+sph_scene* sph_scene_load(const char* path, const sph_settings* s, const char* base_dir) {
+    sph_scene* out = nullptr;
+    int rc = guard([&] {
+        validate(s);
+        if (!path) throw Error(SPH_ERR_INVALID, "scene path is NULL");
+        std::unique_ptr<sph_scene> sc(new sph_scene());
+        sc->run.reset(new SceneRun(path, to_settings(s), base_dir ? base_dir : ""));
+        sc->handle.sim = std::move(sc->run->sim_owner());
+        sc->world.world = &sc->run->world();
+        out = sc.release();
+        return SPH_OK;
+    });
+    return rc == SPH_OK ? out : nullptr;
+}
+
+int sph_scene_destroy(sph_scene* sc) {
+    return guard([&] {
+        delete sc;
+        return SPH_OK;
+    });
+}
+
+sph_handle* sph_scene_simulation(sph_scene* sc) {
+    sph_handle* out = nullptr;
+    guard([&] {
+        out = &scene_of(sc)->handle;
+        return SPH_OK;
+    });
+    return out;
+}
+
+sph_rigid_world* sph_scene_world(sph_scene* sc) {
+    sph_rigid_world* out = nullptr;
+    guard([&] {
+        out = &scene_of(sc)->world;
+        return SPH_OK;
+    });
+    return out;
+}
+
+int sph_scene_step(sph_scene* sc, int nsteps) {
+    return guard([&] {
+        if (nsteps < 0) throw Error(SPH_ERR_RANGE, "negative step count");
+        for (int k = 0; k < nsteps; ++k) scene_of(sc)->run->step();
+        return SPH_OK;
+    });
+}
+
+int sph_scene_body_count(sph_scene* sc) {
+    return guard([&] { return (int)scene_of(sc)->run->bodies().size(); });
+}
+
+int sph_scene_body(sph_scene* sc, int k, char* type, char* name, int* sph_id, int* rigid_id, float* mass,
+                   int* samples) {
+    return guard([&] {
+        const auto& b = scene_of(sc)->run->bodies();
+        if (k < 0 || k >= (int)b.size()) throw Error(SPH_ERR_RANGE, "no scene body " + std::to_string(k));
+        const auto& e = b[k];
+        if (type) {
+            std::strncpy(type, e.body.type.c_str(), 15);
+            type[15] = 0;
+        }
+        if (name) {
+            std::strncpy(name, e.body.name.c_str(), 63);
+            name[63] = 0;
+        }
+        if (sph_id) *sph_id = e.sph_id;
+        if (rigid_id) *rigid_id = e.rigid_id;
+        if (mass) *mass = e.body.mass;
+        if (samples) *samples = e.samples;
         return SPH_OK;
     });
 }

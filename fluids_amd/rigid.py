@@ -32,9 +32,18 @@ class RigidWorld:
         if not self._h:
             raise RuntimeError(L.sph_last_error().decode())
 
+    @classmethod
+    def _borrow(cls, handle):
+        """A view of a world owned elsewhere (a loaded scene); close() leaves it."""
+        obj = cls.__new__(cls)
+        obj._h = handle
+        obj._borrowed = True
+        return obj
+
     def close(self):
         if getattr(self, "_h", None):
-            lib().sph_rigid_destroy(self._h)
+            if not getattr(self, "_borrowed", False):
+                lib().sph_rigid_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -11,7 +11,7 @@ from dataclasses import asdict, dataclass, field
 
 import numpy as np
 
-from ._lib import (FIELDS, I32_FIELDS, LIST_CAP, PCISPH, U32_FIELDS, VEC4_FIELDS, WCSPH,
+from ._lib import (SphError, FIELDS, I32_FIELDS, LIST_CAP, PCISPH, U32_FIELDS, VEC4_FIELDS, WCSPH,
                    SphSettings, SphStepStats, check, lib)
 
 
@@ -99,9 +99,19 @@ class FluidSimulation:
         if not self._h:
             raise RuntimeError("sph_create failed: " + lib().sph_last_error().decode())
 
+    @classmethod
+    def _borrow(cls, handle, settings: SphSettings):
+        """A view of a solver owned elsewhere (a loaded scene); close() leaves it."""
+        obj = cls.__new__(cls)
+        obj._settings = settings
+        obj._h = handle
+        obj._borrowed = True
+        return obj
+
     def close(self):
         if getattr(self, "_h", None):
-            lib().sph_destroy(self._h)
+            if not getattr(self, "_borrowed", False):
+                lib().sph_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -283,3 +293,63 @@ def cube_sampling(side, spacing) -> np.ndarray:
 def wall_sampling(width, height, spacing) -> np.ndarray:
     """Wall::surface_sampling (scene/wall.cpp:142-152)."""
     return _sample(lib().sph_sample_wall, float(width), float(height), float(spacing))
+
+
+def sphere_sampling(radius, spacing) -> np.ndarray:
+    """Sphere::surface_sampling (scene/sphere.cpp:135-174), body-local."""
+    return _sample(lib().sph_sample_sphere, float(radius), float(spacing))
+
+
+def model_sampling(obj_path, spacing) -> np.ndarray:
+    """Model::surface_sampling (scene/model.cpp:170-214): voxel corners of an
+    OBJ mesh, body-local."""
+    return _sample(lib().sph_sample_model, str(obj_path).encode(), float(spacing))
+
+
+class LoadedScene:
+    """Scene::load_scene (scene/scene.cpp:322-476) without rendering: the
+    solver, the rigid world and the bodies of a scene file, advanced frame by
+    frame as Scene::render does (scene.cpp:100-110)."""
+
+    def __init__(self, path, physics: PhysicsSettings, sim: SimulationSettings, base_dir=None):
+        from .rigid import RigidWorld
+        self._settings = make_settings(physics, sim)
+        L = lib()
+        self._h = L.sph_scene_load(str(path).encode(), C.byref(self._settings),
+                                   str(base_dir).encode() if base_dir else None)
+        if not self._h:
+            raise SphError(-5, L.sph_last_error().decode())
+        cls = WCSPHSimulation if sim.sim_method == WCSPH else PCISPHSimulation
+        self.simulation = cls._borrow(L.sph_scene_simulation(self._h), self._settings)
+        self.world = RigidWorld._borrow(L.sph_scene_world(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.simulation._h = None
+            self.world._h = None
+            lib().sph_scene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n: int = 1):
+        check(lib().sph_scene_step(self._h, int(n)))
+
+    def bodies(self) -> list:
+        out = []
+        for k in range(check(lib().sph_scene_body_count(self._h))):
+            t, nm = C.create_string_buffer(16), C.create_string_buffer(64)
+            sid, rid, smp = C.c_int(), C.c_int(), C.c_int()
+            m = C.c_float()
+            check(lib().sph_scene_body(self._h, k, t, nm, C.byref(sid), C.byref(rid), C.byref(m), C.byref(smp)))
+            out.append({"type": t.value.decode(), "name": nm.value.decode(), "boundary_id": sid.value,
+                        "rigid_id": rid.value, "mass": m.value, "samples": smp.value})
+        return out
+
+
+def load_scene(path, physics: PhysicsSettings, sim: SimulationSettings, base_dir=None) -> LoadedScene:
+    return LoadedScene(path, physics, sim, base_dir)

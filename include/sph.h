@@ -252,6 +252,11 @@ int sph_sample_fishtank(float width, float height, float depth, float spacing, f
 int sph_sample_cube(float side, float spacing, float* out, int cap);
 /* Wall::surface_sampling (scene/wall.cpp:142-152), body-local. */
 int sph_sample_wall(float width, float height, float spacing, float* out, int cap);
+/* Sphere::surface_sampling (scene/sphere.cpp:135-174) over IcoSphereMesh(radius, 3), body-local. */
+int sph_sample_sphere(float radius, float spacing, float* out, int cap);
+/* Model::surface_sampling (scene/model.cpp:170-214): voxel corners of an OBJ
+ * mesh (mesh/modelmesh.cpp) at the given spacing, body-local. */
+int sph_sample_model(const char* obj_path, float spacing, float* out, int cap);
 
 /* ------------------------------------------------------- rigid bodies (F1)
  * A host rigid-body world in place of Bullet's btDiscreteDynamicsWorld for
@@ -377,6 +382,29 @@ int sph_adjoint_backward(sph_adjoint* a, const float* g_pos, const float* g_vel,
                          float* g_pos0, float* g_vel0, float* g_vel_half0, double* g_params);
 /* Wall-clock milliseconds of the last forward and backward passes. */
 int sph_adjoint_timings(sph_adjoint* a, double* forward_ms, double* backward_ms);
+
+/* ------------------------------------------------------ scene files (F2)
+ * Scene::load_scene (scene/scene.cpp:322-476) without rendering: reads a
+ * scene file (container, box fluid volumes, cube / sphere / model / wall
+ * bodies), builds the solver in the reference's init order, samples each
+ * body at twice the particle radius (boundaryhandler.cpp:19) as a boundary
+ * that moves when its mass is > 0, and puts the bodies and a FishTank
+ * container into a rigid world. base_dir resolves relative model paths. */
+typedef struct sph_scene sph_scene;
+
+sph_scene* sph_scene_load(const char* path, const sph_settings* s, const char* base_dir);
+int sph_scene_destroy(sph_scene* sc);
+/* The scene's solver and rigid world, owned by the scene (do not destroy). */
+sph_handle* sph_scene_simulation(sph_scene* sc);
+sph_rigid_world* sph_scene_world(sph_scene* sc);
+/* Frames of Scene::render (scene.cpp:100-110): sph_coupled_step over every
+ * body with mass > 0. */
+int sph_scene_step(sph_scene* sc, int nsteps);
+int sph_scene_body_count(sph_scene* sc);
+/* Body k: type (16 chars), name (64 chars), its fluid boundary id, its rigid
+ * body id, mass and sample count. Outputs may be NULL. */
+int sph_scene_body(sph_scene* sc, int k, char* type, char* name, int* sph_id, int* rigid_id, float* mass,
+                   int* samples);
 
 #ifdef __cplusplus
 }
