@@ -100,6 +100,12 @@ SIGNATURES = {
     "sph_scene_step": (I, [P, I]),
     "sph_scene_body_count": (I, [P]),
     "sph_scene_body": (I, [P, I, P, P, P, P, P, P]),
+    "sph_frames_open": (P, [P, C.c_char_p, I]),
+    "sph_frames_write": (I, [P]),
+    "sph_frames_close": (I, [P]),
+    "sph_perf_open": (P, [P, C.c_char_p]),
+    "sph_perf_frame": (I, [P]),
+    "sph_perf_close": (I, [P]),
     "sph_rigid_create": (P, [F]),
     "sph_rigid_destroy": (I, [P]),
     "sph_rigid_set_container": (I, [P, F, F, F, P]),

@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsph.so")
 
-SOURCES = ["grid.hip", "pcisph.hip", "wcsph.hip", "boundary.hip", "adjoint.hip", "adjoint.cpp", "solver.cpp", "mesh_samplers.cpp", "scene.cpp", "rigid.cpp", "batch.cpp", "sph_abi.cpp"]
+SOURCES = ["grid.hip", "pcisph.hip", "wcsph.hip", "boundary.hip", "adjoint.hip", "adjoint.cpp", "solver.cpp", "mesh_samplers.cpp", "scene.cpp", "output.cpp", "rigid.cpp", "batch.cpp", "sph_abi.cpp"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

@@ -949,6 +949,8 @@ WcStepParams wcsph_step_params(const Settings& s, const float container[3]) {
 // WCSPHSimulation::simulate (wcsphsimluation.cpp:40-106)
 void WCSPHSimulation::simulate() {
     require_ready();
+    ++frames_done;
+    sim_time += s_.time_step;
     ensure_ranges();
     hipStream_t s = stream_;
     launch_step_begin(d_state.get(), s);

@@ -219,6 +219,9 @@ public:
     DeviceArray<DevState> d_state;
     DeviceArray<unsigned long long> d_acc;
     float container[3] = {0, 0, 0};
+    // frames advanced by simulate() and the simulated time they cover
+    long long frames_done = 0;
+    double sim_time = 0.0;
     bool have_container = false;
 
 protected:
@@ -246,7 +249,11 @@ protected:
 class PCISPHSimulation : public SimulationBase {
 public:
     explicit PCISPHSimulation(const Settings& s);
-    void simulate() override { simulate_step(3, s_.pcisph_max_iterations); }
+    void simulate() override {
+        simulate_step(3, s_.pcisph_max_iterations);
+        ++frames_done;
+        sim_time += s_.time_step;
+    }
     float density_scale_factor() const { return delta_; }
     PciParams params() const { return P_; }
 

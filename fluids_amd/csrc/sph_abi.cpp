@@ -8,6 +8,7 @@
 #include "adjoint_host.h"
 #include "batch_host.h"
 #include "rigid.h"
+#include "output.h"
 #include "scene.h"
 #include "solver.h"
 #include "sph.h"
@@ -29,6 +30,14 @@ struct sph_batch {
 struct sph_rigid_world {
     std::unique_ptr<RigidWorld> own;
     RigidWorld* world = nullptr;  // own.get(), or a world owned by a scene
+};
+
+struct sph_frames {
+    std::unique_ptr<FrameWriter> w;
+};
+
+struct sph_perf_log {
+    std::unique_ptr<PerfLog> p;
 };
 
 struct sph_scene {
@@ -859,6 +868,66 @@ int sph_scene_body(sph_scene* sc, int k, char* type, char* name, int* sph_id, in
         if (mass) *mass = e.body.mass;
         if (samples) *samples = e.samples;
         return SPH_OK;
+    });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ output
+extern "C" {
+
+sph_frames* sph_frames_open(sph_handle* h, const char* path, int fields) {
+    sph_frames* out = nullptr;
+    int rc = guard([&] {
+        if (!path) throw Error(SPH_ERR_INVALID, "frames path is NULL");
+        std::unique_ptr<FrameWriter> w(new FrameWriter(sim_of(h), path, fields));
+        out = new sph_frames{std::move(w)};
+        return SPH_OK;
+    });
+    return rc == SPH_OK ? out : nullptr;
+}
+
+int sph_frames_write(sph_frames* f) {
+    return guard([&] {
+        if (!f || !f->w) throw Error(SPH_ERR_INVALID, "frames is NULL");
+        f->w->write();
+        return SPH_OK;
+    });
+}
+
+int sph_frames_close(sph_frames* f) {
+    return guard([&] {
+        if (!f) return SPH_OK;
+        std::unique_ptr<sph_frames> own(f);
+        long long n = f->w ? f->w->close() : 0;
+        return (int)n;
+    });
+}
+
+sph_perf_log* sph_perf_open(sph_handle* h, const char* path) {
+    sph_perf_log* out = nullptr;
+    int rc = guard([&] {
+        if (!path) throw Error(SPH_ERR_INVALID, "performance log path is NULL");
+        std::unique_ptr<PerfLog> p(new PerfLog(sim_of(h), path));
+        out = new sph_perf_log{std::move(p)};
+        return SPH_OK;
+    });
+    return rc == SPH_OK ? out : nullptr;
+}
+
+int sph_perf_frame(sph_perf_log* p) {
+    return guard([&] {
+        if (!p || !p->p) throw Error(SPH_ERR_INVALID, "performance log is NULL");
+        p->p->frame();
+        return SPH_OK;
+    });
+}
+
+int sph_perf_close(sph_perf_log* p) {
+    return guard([&] {
+        long long n = (p && p->p) ? p->p->lines() : 0;
+        delete p;
+        return (int)n;
     });
 }
 

@@ -406,6 +406,34 @@ int sph_scene_body_count(sph_scene* sc);
 int sph_scene_body(sph_scene* sc, int k, char* type, char* name, int* sph_id, int* rigid_id, float* mass,
                    int* samples);
 
+/* ------------------------------------------------------------ output (F4)
+ * In place of the GL vertex buffer the reference renders from
+ * (opencl/clallocator.h:165-183, 300-374) the particle buffers leave the
+ * device as a frame file, and in place of the -o option (main.cpp:19,
+ * 34-35; gui/glwidget.cpp:48-73; gui/mainwindow.cpp:99-101) frame times go
+ * to an FPS log.
+ *
+ * Frame file: "SPHF", u32 version (1), u32 fields (bit 0 positions, bit 1
+ * velocities), u32 0; per frame: u64 frame index, f64 simulated time, u32 n,
+ * u32 fields, n float4 positions then n float4 velocities (as present), in
+ * stored order. Reader: fluids_amd.output.read_frames. */
+typedef struct sph_frames sph_frames;
+typedef struct sph_perf_log sph_perf_log;
+
+sph_frames* sph_frames_open(sph_handle* h, const char* path, int fields);
+/* Queues the current state: an asynchronous device-to-pinned copy on the
+ * handle's stream; a background thread writes it. */
+int sph_frames_write(sph_frames* f);
+/* Writes the queued frames, closes the file, frees f; returns the frame count. */
+int sph_frames_close(sph_frames* f);
+/* FPS log: call sph_perf_frame after each frame (it waits for the handle's
+ * stream); every 6 frames one line with the frames-per-second of those
+ * frames is written, as the reference's widget does. */
+sph_perf_log* sph_perf_open(sph_handle* h, const char* path);
+int sph_perf_frame(sph_perf_log* p);
+/* Closes the log, frees p; returns the number of lines written. */
+int sph_perf_close(sph_perf_log* p);
+
 #ifdef __cplusplus
 }
 #endif
