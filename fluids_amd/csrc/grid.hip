@@ -54,7 +54,7 @@ __global__ __launch_bounds__(256) void k_hash(int n, const float4* __restrict__ 
 // lanes sharing a digit are the AND of 8 bit-ballots), then orders waves
 // through LDS, so the output keeps input order within equal digits.
 constexpr int SORT_THREADS = 256;
-constexpr int SORT_ITEMS = 16;
+constexpr int SORT_ITEMS = 8;
 constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;
 constexpr int RADIX_MAX = 1024;
 
@@ -441,11 +441,55 @@ struct ListSink {
     }
 };
 
+// A buffer view of an array of `bytes` bytes (raw, no stride): loads at an
+// offset past the end read zero. Offsets are 32-bit; the launchers keep the
+// particle counts below MAX_SCENE_PARTICLES so that they fit.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_view(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float4 load4(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff) {
+    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, off, soff, 0));
+}
+__device__ __forceinline__ float load1(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
+}
+
+// The hits of one batch of candidates j0 .. j0 + WALK_BATCH - 1, in order.
+// With CHECKS the batch may run past the cell's end (rem candidates left)
+// or past the list cap; without, the caller has established that neither
+// can happen for any lane of the wave.
+template <bool CHECKS, bool WCSPH, bool PHI, class Sink>
+__device__ __forceinline__ void batch_hits(V3 pi, int j0, int rem, const float4 (&q)[WALK_BATCH],
+                                           const float (&w)[WALK_BATCH], const GridDev& g, const DensityParams& P,
+                                           const Sink& list, int& len, float& d) {
+#pragma unroll
+    for (int u = 0; u < WALK_BATCH; ++u) {
+        V3 r = pi - v3(q[u]);
+        float r2 = dot(r, r);
+        bool hit = r2 < g.sqr_support;
+        if (CHECKS) hit = hit && u < rem && len < LIST_CAP;
+        if (hit) {
+            list.put(len, j0 + u);
+            ++len;
+            accumulate_density<WCSPH>(r2, d, P, w[u], PHI);
+        }
+    }
+}
+
+// The walk over the 27 cells. Candidates are loaded WALK_BATCH at a time
+// through a buffer view, from one base offset per batch (the per-slot offset
+// is a scalar operand); loads past the cell's end read neighbours or zero and
+// are never used. The distance of every slot is computed unconditionally;
+// the range, cap and radius tests only guard the list entry and the density
+// term, and the range and cap tests are skipped for batches where the whole
+// wave has a full batch left and room for it under the cap.
 template <bool WCSPH, bool PHI, class Sink>
 __device__ __forceinline__ int walk_cells(V3 pi, int cx, int cy, int cz, const int2* __restrict__ R,
                                           const float4* __restrict__ P4, const float* __restrict__ phi,
                                           int count, const GridDev& g, const DensityParams& P,
                                           const Sink& list, float& d, int& edge) {
+    const __amdgpu_buffer_rsrc_t rp = buffer_view(P4, (uint32_t)count * 16u);
+    const __amdgpu_buffer_rsrc_t rw = buffer_view(phi, PHI ? (uint32_t)count * 4u : 0u);
     int len = 0;
     int2 cur = cell_range(R, cx, cy, cz, g, edge);  // own cell first
     for (int o = 0; o < 27 && len < LIST_CAP; ++o) {
@@ -458,27 +502,18 @@ __device__ __forceinline__ int walk_cells(V3 pi, int cx, int cy, int cz, const i
         }
         const int a = cur.x, b = cur.y < count ? cur.y : count;  // ranges never exceed count
         for (int j0 = a; j0 < b && len < LIST_CAP; j0 += WALK_BATCH) {
+            const int rem = b - j0;
             float4 q[WALK_BATCH];
             float w[WALK_BATCH];
 #pragma unroll
             for (int u = 0; u < WALK_BATCH; ++u) {
-                int j = j0 + u < b ? j0 + u : b - 1;
-                q[u] = P4[j];
-                w[u] = PHI ? phi[j] : 1.0f;
+                q[u] = load4(rp, (uint32_t)j0 * 16u, 16u * u);
+                w[u] = PHI ? load1(rw, (uint32_t)j0 * 4u, 4u * u) : 1.0f;
             }
-#pragma unroll
-            for (int u = 0; u < WALK_BATCH; ++u) {
-                int j = j0 + u;
-                if (j < b && len < LIST_CAP) {
-                    V3 r = pi - v3(q[u]);
-                    float r2 = dot(r, r);
-                    if (r2 < g.sqr_support) {
-                        list.put(len, j);
-                        ++len;
-                        accumulate_density<WCSPH>(r2, d, P, w[u], PHI);
-                    }
-                }
-            }
+            if (__all(rem >= WALK_BATCH && len <= LIST_CAP - WALK_BATCH))
+                batch_hits<false, WCSPH, PHI>(pi, j0, rem, q, w, g, P, list, len, d);
+            else
+                batch_hits<true, WCSPH, PHI>(pi, j0, rem, q, w, g, P, list, len, d);
         }
         cur = nxt;
     }

@@ -137,29 +137,42 @@ __global__ __launch_bounds__(256) void k_update_pressure(int n, int iter, const 
         V3 xi = v3(pos_pred[i]);
         float pd = 0.0f, pdb = 0.0f;
         const int len = nc[i];
-        const int* L = nl + i;
-        for (int k0 = 0; k0 < len; k0 += GATHER_BATCH) {
-            int jj[GATHER_BATCH];
+        const int rows = wave_max_small(len);
+        const int dflt = wave_first(i);
+        int jj[GATHER_BATCH];
+        if (rows > 0) list_batch(nl, n, i, len, rows, 0, dflt, jj);
+        for (int k0 = 0; k0 < rows; k0 += GATHER_BATCH) {
             float4 q[GATHER_BATCH];
 #pragma unroll
-            for (int u = 0; u < GATHER_BATCH; ++u) jj[u] = k0 + u < len ? L[(size_t)(k0 + u) * n] : i;
-#pragma unroll
             for (int u = 0; u < GATHER_BATCH; ++u) q[u] = pos_pred[jj[u]];
+            if (k0 + GATHER_BATCH < rows) list_batch(nl, n, i, len, rows, k0 + GATHER_BATCH, dflt, jj);
 #pragma unroll
             for (int u = 0; u < GATHER_BATCH; ++u) {
-                if (k0 + u < len) {
-                    V3 r = xi - v3(q[u]);
-                    pd += cube(P.H2 - dot(r, r));
-                }
+                V3 r = xi - v3(q[u]);
+                float t = pd + cube(P.H2 - dot(r, r));
+                if (k0 + u < len) pd = t;
             }
         }
         pd *= P.w_eval * P.M;
         if (has_b) {
-            int blen = bnc[i];
-            for (int k = 0; k < blen; ++k) {
-                int j = bnl[(size_t)k * n + i];
-                V3 r = xi - v3(bpos_s[j]);
-                pdb += cube(P.H2 - dot(r, r)) * bphi_s[j];
+            const int blen = bnc[i];
+            const int brows = wave_max_small(blen);
+            for (int k0 = 0; k0 < brows; k0 += GATHER_BATCH) {
+                int jj[GATHER_BATCH];
+                float4 q[GATHER_BATCH];
+                float w[GATHER_BATCH];
+                list_batch(bnl, n, i, blen, brows, k0, 0, jj);  // past blen: boundary particle 0, unused
+#pragma unroll
+                for (int u = 0; u < GATHER_BATCH; ++u) {
+                    q[u] = bpos_s[jj[u]];
+                    w[u] = bphi_s[jj[u]];
+                }
+#pragma unroll
+                for (int u = 0; u < GATHER_BATCH; ++u) {
+                    V3 r = xi - v3(q[u]);
+                    float t = pdb + cube(P.H2 - dot(r, r)) * w[u];
+                    if (k0 + u < blen) pdb = t;
+                }
             }
             pdb *= P.w_eval;
         }

@@ -294,6 +294,8 @@ void BoundaryHandler::resample() {
     hipStream_t s = sim_.stream_;
     const Settings& S = sim_.s_;
     int B = count();
+    if (B > MAX_SCENE_PARTICLES)
+        throw Error(SPH_ERR_RANGE, "at most " + std::to_string(MAX_SCENE_PARTICLES) + " boundary particles per scene");
     moving_ = 0;
     for (auto& b : bodies_) moving_ += b.can_move ? 1 : 0;
     d_raw.resize(B);
@@ -489,6 +491,8 @@ void SimulationBase::reset(const Settings& s) {
 }
 
 void SimulationBase::alloc_fluid(int n) {
+    if (n > MAX_SCENE_PARTICLES)
+        throw Error(SPH_ERR_RANGE, "at most " + std::to_string(MAX_SCENE_PARTICLES) + " fluid particles per scene");
     n_ = n;
     size_t N = (size_t)std::max(n, 1);
     d_pos.resize(N); d_vel.resize(N); d_pos_s.resize(N); d_vel_s.resize(N);

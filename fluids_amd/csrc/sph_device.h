@@ -24,6 +24,10 @@ namespace sph {
 constexpr int LIST_CAP = 50;      // NEIGH_LIST_MAX_LENGTH (kernels/common.h:4)
 constexpr int MAXERR_SLOTS = 16;  // ring of per-correction max density errors
 constexpr int GATHER_BATCH = 8;  // neighbour loads issued together per thread
+// Largest fluid or boundary particle count of one scene (or one batch): the
+// neighbour walk reads candidates through buffer views with 32-bit byte
+// offsets.
+constexpr int MAX_SCENE_PARTICLES = 1 << 27;
 
 // Uniform grid restated from kernels/grid.h and fluid/simulation/grid.cpp:61-68.
 // Cell ranges live in a dense box: all cps cells in y (the 5 m grid is
@@ -178,5 +182,38 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     int q = nb >> 3, r = nb & 7, x = b & 7;
     return x * q + (x < r ? x : r) + (b >> 3);
 }
+
+// Largest of v (0 <= v < 64) over the active lanes of the wave; every lane
+// gets the same value.
+__device__ __forceinline__ int wave_max_small(int v) {
+    int m = 0;
+#pragma unroll
+    for (int b = 5; b >= 0; --b) {
+        const int c = m | (1 << b);
+        if (__ballot(v >= c)) m = c;
+    }
+    return m;
+}
+
+// Entries k0 .. k0 + B - 1 of particle i's column-major list
+// (list[k * stride + i]); an entry at or past len reads as dflt. `rows` is
+// the wave's largest len and k0 < rows, so every row read is one that some
+// lane of the wave uses (rows past it re-read row rows - 1). The loads carry
+// no branch, so all B of them are in flight together; the row offset is the
+// same for the whole wave. A dflt shared by the wave keeps the gathers of
+// unused entries on one cache line.
+template <int B>
+__device__ __forceinline__ void list_batch(const int* __restrict__ list, size_t stride, int i, int len, int rows,
+                                           int k0, int dflt, int (&jj)[B]) {
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+        const int k = k0 + u < rows ? k0 + u : rows - 1;
+        const int j = list[(size_t)k * stride + i];
+        jj[u] = k0 + u < len ? j : dflt;
+    }
+}
+
+// An index every active lane of the wave holds the same value of.
+__device__ __forceinline__ int wave_first(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 }  // namespace sph
