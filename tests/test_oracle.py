@@ -169,6 +169,18 @@ def test_neighbour_lists_equal_brute_force_in_reference_order():
         assert list(nl[: nc[i], i]) == want[i], i
 
 
+def test_grid_path_at_full_size_against_restatement():
+    """The 1M-particle throughput scene after two steps: keys, permutation,
+    a sample of lists and of densities against the numpy restatement in
+    helpers (the same check the GPU runs at 4M)."""
+    from helpers import check_grid_properties
+    sc = scenes.c3_pcisph_throughput(1_000_000)
+    o = make(sc)
+    o.step(2)
+    lists, dens = check_grid_properties(o.field, o.constants(), sc.settings["fluid_support_radius"], 600, 17)
+    assert lists == 600 and dens > 500
+
+
 def test_list_cap_truncates_at_fifty():
     """Q4: 60 particles packed into one cell; every list stops at 50 entries,
     taking the lowest indices of the own cell first."""
