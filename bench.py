@@ -31,8 +31,10 @@ from fluids_amd.profiling import read_traffic  # noqa: E402
 
 
 def kernel_bytes(name, K, Kb, n_it):
-    """Algorithmic bytes per particle of one launch of each kernel (the
-    reference kernels each one replaces)."""
+    """Algorithmic bytes per particle of one launch of each kernel: the
+    unique bytes of the reference kernels that the launch replaces, with
+    list indices as int32 at the measured mean neighbour counts K and Kb and
+    n_it PCISPH corrections per step."""
     table = {
         "hash": 24.0,
         "radix_scatter": 16.0 / 4,  # the reference's sort counts 16 B/p over all passes
@@ -169,6 +171,7 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=50)
     ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
+    ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint timing")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "pmc_pcisph_throughput_1000k.md"),
                     help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary (python -m fluids_amd.profiling pmc) "
                          "that gives the dominant kernel's HBM bytes per launch")
@@ -176,12 +179,16 @@ def main():
 
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the solver has no multi-GPU path. "
-              "See DESIGN.md for how a scene would shard.", file=sys.stderr)
+              "A scene would shard into Morton-contiguous slabs with halo exchanges; this build "
+              "measures one GPU.", file=sys.stderr)
         sys.exit(2)
 
     from fluids_amd import scenes
     from fluids_amd._lib import lib
     lib()
+    if a.adjoint_only:
+        print(json.dumps({"adjoint": run_adjoint()}), flush=True)
+        return
     if a.batched_only:
         print(json.dumps({"batched": run_batched(a.batched_steps, 5, a.plain_steps),
                           "adjoint": None if a.no_adjoint else run_adjoint()}), flush=True)

@@ -572,7 +572,7 @@ __device__ __forceinline__ void lists_density_one(
 }
 
 template <bool WCSPH>
-__global__ __launch_bounds__(LISTS_THREADS) void k_lists_density(
+__global__ __launch_bounds__(LISTS_THREADS, 8) void k_lists_density(
     int n, const float4* __restrict__ pos_s, const int2* __restrict__ fR, int nb, const float4* __restrict__ bpos_s,
     const float* __restrict__ bphi_s, const int2* __restrict__ bR, const uint8_t* __restrict__ bnear, GridDev g,
     DensityParams P, int* __restrict__ nl, int* __restrict__ nc, int* __restrict__ bnl, int* __restrict__ bnc,

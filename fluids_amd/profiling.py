@@ -16,6 +16,7 @@ Usage::
     python -m fluids_amd.profiling trace <trace_dir> [--traffic traffic.md] > summary.md
     python -m fluids_amd.profiling pmc <fetch_dir> <write_dir> <copy_fetch_dir> <copy_write_dir> \
         --copy-bytes 268435456 [--tcc-dir <hit_miss_dir>] > traffic.md
+    python -m fluids_amd.profiling stats <trace_dir or *kernel_stats.csv> > stats.md
 
 ``bench.py`` reads the machine-readable ``traffic`` rows of a ``pmc`` summary
 (``read_traffic``) to fill its roofline's ``traffic`` field.
@@ -161,6 +162,22 @@ def copy_calibration(nbytes):
     print(f"copied {nbytes} B per launch", flush=True)
 
 
+def stats_summary(stats_csv, top=25):
+    """The rows of rocprofv3's --stats kernel table (all dispatches), by total time."""
+    rows = list(csv.DictReader(open(_csv(stats_csv, "kernel_stats.csv"))))
+    agg = collections.OrderedDict()
+    for r in rows:
+        k = short_name(r["Name"])
+        a = agg.setdefault(k, [0, 0.0, 0.0])
+        a[0] += int(r["Calls"])
+        a[1] += float(r["TotalDurationNs"])
+        a[2] += float(r["Percentage"])
+    out = ["| kernel | calls | total ms | avg us | % |", "|---|---|---|---|---|"]
+    for k, (calls, tot, pct) in sorted(agg.items(), key=lambda kv: -kv[1][1])[:top]:
+        out.append(f"| {k} | {calls} | {tot / 1e6:.3f} | {tot / calls / 1e3:.1f} | {pct:.1f} |")
+    return "\n".join(out)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -176,11 +193,15 @@ def main(argv=None):
     p.add_argument("copy_write_dir")
     p.add_argument("--copy-bytes", type=int, required=True)
     p.add_argument("--tcc-dir", default=None, help="run with TCC_HIT_sum and TCC_MISS_sum, for L2 hit rates")
+    st = sub.add_parser("stats")
+    st.add_argument("stats_csv", help="a *kernel_stats.csv or a directory holding one")
     a = ap.parse_args(argv)
     if a.cmd == "copy":
         copy_calibration(a.bytes)
     elif a.cmd == "trace":
         print(trace_summary(a.trace_dir, a.traffic))
+    elif a.cmd == "stats":
+        print(stats_summary(a.stats_csv))
     else:
         print(pmc_summary(a.fetch_dir, a.write_dir, a.copy_fetch_dir, a.copy_write_dir, a.copy_bytes, a.tcc_dir))
 
