@@ -399,9 +399,10 @@ void launch_mark_boundary(int n, const float4* bpos_s, const uint32_t* keys_s, c
 // order, sorted order inside a cell, strict r2 < CELL_SIZE^2, cap 50,
 // column-major list[k*N + i]. The density kernels read exactly these r2 in
 // exactly this order, so the sum is accumulated while the list is written.
-// The 27 cell ranges are fetched up front and candidates are loaded four at
-// a time, so several loads are in flight per thread; they are consumed in
-// order, which keeps both the list and the sum identical.
+// Each cell's range is fetched while the previous cell is processed, and a
+// cell's candidates are loaded WALK_BATCH at a time, so several loads are in
+// flight per thread; they are consumed in order, which keeps both the list
+// and the sum identical.
 template <bool WCSPH>
 __device__ __forceinline__ void accumulate_density(float r2, float& d, const DensityParams& P, float w,
                                                    bool weighted) {
