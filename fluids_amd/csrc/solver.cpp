@@ -599,22 +599,24 @@ void SimulationBase::synchronize() { hip_check(hipStreamSynchronize(stream_), "h
 
 void SimulationBase::get_state(int which, float* xyzw, int n) {
     if (n != n_) throw Error(SPH_ERR_RANGE, "size mismatch: " + std::to_string(n) + " vs " + std::to_string(n_));
+    if (which == 2 && s_.method != SPH_WCSPH) throw Error(SPH_ERR_STATE, "buffer not available for this method");
+    if (n == 0) return;  // a scene without fluid particles has no buffers yet
     DeviceArray<float4>* src = which == 0 ? &d_pos : which == 1 ? &d_vel : &d_vh;
     if (!src->get()) throw Error(SPH_ERR_STATE, "buffer not available for this method");
-    if (n == 0) return;
     hip_check(hipMemcpyAsync(xyzw, src->get(), n * sizeof(float4), hipMemcpyDeviceToHost, stream_), "download");
     synchronize();
 }
 
 void SimulationBase::set_state(int which, const float* xyzw, int n) {
     if (n != n_) throw Error(SPH_ERR_RANGE, "size mismatch: " + std::to_string(n) + " vs " + std::to_string(n_));
+    if (which == 2 && s_.method != SPH_WCSPH) throw Error(SPH_ERR_STATE, "buffer not available for this method");
+    if (n == 0) return;  // a scene without fluid particles has no buffers yet
     DeviceArray<float4>* dst = which == 0 ? &d_pos : which == 1 ? &d_vel : &d_vh;
     if (!dst->get()) throw Error(SPH_ERR_STATE, "buffer not available for this method");
     std::vector<float4> v(n);
     for (int i = 0; i < n; ++i)
         v[i] = make_float4(xyzw[4 * i], xyzw[4 * i + 1], xyzw[4 * i + 2], which == 0 ? 1.0f : 0.0f);
     if (which == 0) ensure_box(v);
-    if (n == 0) return;
     hip_check(hipMemcpyAsync(dst->get(), v.data(), n * sizeof(float4), hipMemcpyHostToDevice, stream_), "upload");
     synchronize();
 }
