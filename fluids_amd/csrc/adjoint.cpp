@@ -39,7 +39,7 @@ WcAdjoint::WcAdjoint(const Settings& s, const float container[3], const float* p
     d_g[0].resize(3 * N); d_g[1].resize(3 * N);
     d_w_abar.resize(N); d_w_xbar.resize(N); d_w_vbar.resize(N); d_w_vhbar.resize(N); d_w_nbar.resize(N);
     d_w_rbar.resize(N); d_w_pbar.resize(N);
-    d_part_a.resize(3 * (size_t)adj_blocks(n_)); d_part_b.resize(3 * (size_t)adj_blocks(n_));
+    d_part_a.resize(3 * (size_t)adj_split_blocks(n_)); d_part_b.resize(3 * (size_t)adj_split_blocks(n_));
     d_params.resize(3); d_capped.resize(1);
     hip_check(hipMemsetAsync(d_bnc.get(), 0, N * sizeof(int), stream_), "memset");
     hip_check(hipMemsetAsync(d_keys_a.get(), 0, N * sizeof(uint32_t), stream_), "memset");

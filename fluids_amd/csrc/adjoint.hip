@@ -76,6 +76,23 @@ __device__ __forceinline__ void block_sum3(float a, float b, float c, float* out
 }  // namespace
 
 constexpr int ADJ_THREADS = 256;
+// The neighbour loops of the adjoint kernels run on ADJ_SPLIT lanes per
+// particle (entries k = part, part + ADJ_SPLIT, ...), whose sums are then
+// combined: at the few tens of thousands of particles an adjoint runs on,
+// one lane per particle leaves most of the GPU idle while each lane walks its
+// list. (The adjoints are accurate to fp32 rounding; unlike the forward pass
+// they are not bit-for-bit tied to an order of summation.)
+constexpr int ADJ_SPLIT = 4;
+
+__device__ __forceinline__ float split_sum(float v) {
+    for (int off = 1; off < ADJ_SPLIT; off <<= 1) v += __shfl_xor(v, off, ADJ_SPLIT);
+    return v;
+}
+__device__ __forceinline__ void split_sum(V3& v) {
+    v.x = split_sum(v.x);
+    v.y = split_sum(v.y);
+    v.z = split_sum(v.z);
+}
 
 // ---------------------------------------------------------- K1: integrate
 // Adjoint of predict_vel_n_pos (wcsph.cl:194-223) for sorted particle i.
@@ -152,18 +169,20 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_accel(int n, AdjTape T, WcP
                                                            float4* __restrict__ xbar, float4* __restrict__ vbar,
                                                            float* __restrict__ rbar, float* __restrict__ pbar,
                                                            float4* __restrict__ nbar, float* __restrict__ partial) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t / ADJ_SPLIT, part = t % ADJ_SPLIT;
     float g_kv = 0.0f, g_sig = 0.0f;
+    V3 gx = zero3(), gv = zero3(), gn = zero3();
+    float gr = 0.0f, gp = 0.0f, gC = 0.0f, di = 1.0f, Ci = 0.0f;
     if (i < n) {
         const float h = P.h, M = P.M, sm = P.st_main, sig = P.sigma, cv = P.visc_lapl * P.k_visc;
         float4 pi4 = T.prho[i];
         V3 xi = v3(pi4), vi = v3(T.vel_s[i]), ni = v3(T.normal[i]), ai = v3(abar[i]);
-        float di = pi4.w, pi = T.pressure[i];
-        float Ci = pi / sqr(di);
-        V3 gx = zero3(), gv = zero3(), gn = zero3();
-        float gr = 0.0f, gp = 0.0f, gC = 0.0f;
+        di = pi4.w;
+        const float pi = T.pressure[i];
+        Ci = pi / sqr(di);
         const int len = T.nc[i];
-        for (int k = 0; k < len; ++k) {
+        for (int k = part; k < len; k += ADJ_SPLIT) {
             int j = T.nl[(size_t)k * n + i];
             float4 pj4 = T.prho[j];
             float dj = pj4.w;
@@ -227,6 +246,14 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_accel(int n, AdjTape T, WcP
                 gr += gK * dK;
             }
         }
+    }
+    split_sum(gx);
+    split_sum(gv);
+    split_sum(gn);
+    gr = split_sum(gr);
+    gp = split_sum(gp);
+    gC = split_sum(gC);
+    if (i < n && part == 0) {
         gp += gC / sqr(di);
         gr += -2.0f * Ci * gC / di;
         xbar[i] = f4(v3(xbar[i]) + gx, 0.0f);
@@ -246,16 +273,17 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_normals(int n, AdjTape T, f
                                                              const float* __restrict__ pbar,
                                                              float4* __restrict__ xbar, float* __restrict__ rbar,
                                                              float* __restrict__ partial) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t / ADJ_SPLIT, part = t % ADJ_SPLIT;
     float g_kg = 0.0f;
+    V3 gx = zero3();
+    float gr = 0.0f, di = 1.0f;
     if (i < n) {
         float4 pi4 = T.prho[i];
         V3 xi = v3(pi4), gni = v3(nbar[i]);
-        float di = pi4.w;
-        V3 gx = zero3();
-        float gr = 0.0f;
+        di = pi4.w;
         const int len = T.nc[i];
-        for (int k = 0; k < len; ++k) {
+        for (int k = part; k < len; k += ADJ_SPLIT) {
             int j = T.nl[(size_t)k * n + i];
             if (j == i) continue;  // r = x_i - x_i does not depend on x_i
             float4 pj4 = T.prho[j];
@@ -269,6 +297,10 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_normals(int n, AdjTape T, f
             gx = gx - (cn / di) * (sqr(w) * gnj - 4.0f * w * dot(r, gnj) * r);
             gr += cn * sqr(w) * dot(r, gnj) / sqr(di);
         }
+    }
+    split_sum(gx);
+    gr = split_sum(gr);
+    if (i < n && part == 0) {
         float p = T.pressure[i];
         if (p > 0.0f) {
             float x = di / rho0;
@@ -292,15 +324,15 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_density(int n, AdjTape T, f
                                                              const float4* __restrict__ vhbar,
                                                              float4* __restrict__ gx_out, float4* __restrict__ gv_out,
                                                              float4* __restrict__ gvh_out, int* __restrict__ capped) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t / ADJ_SPLIT, part = t % ADJ_SPLIT;
+    V3 gx = zero3();
+    if (i < n) {
     float4 pi4 = T.prho[i];
     V3 xi = v3(pi4);
     float gri = rbar[i];
-    V3 gx = v3(xbar[i]);
     const int len = T.nc[i];
-    if (len >= LIST_CAP) atomicAdd(capped, 1);
-    for (int k = 0; k < len; ++k) {
+    for (int k = part; k < len; k += ADJ_SPLIT) {
         int j = T.nl[(size_t)k * n + i];
         if (j == i) continue;
         V3 r = xi - v3(T.prho[j]);
@@ -314,6 +346,11 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_density(int n, AdjTape T, f
             if (q > 0.0f) gx = gx + (-3.0f * c_rho * sqr(h2 - q) * g / q) * r;
         }
     }
+    }
+    split_sum(gx);
+    if (i >= n || part != 0) return;
+    if (T.nc[i] >= LIST_CAP) atomicAdd(capped, 1);
+    gx = v3(xbar[i]) + gx;
     int s = T.perm[i];
     gx_out[s] = f4(gx, 0.0f);
     gv_out[s] = vbar[i];
@@ -341,22 +378,23 @@ __global__ void k_adj_ids(int n, const int* __restrict__ perm, const int* __rest
 }
 
 int adj_blocks(int n) { return (n + ADJ_THREADS - 1) / ADJ_THREADS; }
+int adj_split_blocks(int n) { return (int)(((long long)n * ADJ_SPLIT + ADJ_THREADS - 1) / ADJ_THREADS); }
 
 void launch_adjoint_step(int n, const AdjTape& T, const WcParams& P, float cn, float nH2, float c_rho, float h2,
                          int fix, float rho0, float k_gas, const float4* gx, const float4* gv, const float4* gvh,
                          AdjWork& W, float4* gx_out, float4* gv_out, float4* gvh_out, double* params, int* capped,
                          hipStream_t s) {
     if (n <= 0) return;
-    const int nb = adj_blocks(n);
+    const int nb = adj_blocks(n), ns = adj_split_blocks(n);
     hipLaunchKernelGGL(k_adj_integrate, dim3(nb), dim3(ADJ_THREADS), 0, s, n, T, P, gx, gv, gvh, W.abar, W.xbar,
                        W.vhbar);
-    hipLaunchKernelGGL(k_adj_accel, dim3(nb), dim3(ADJ_THREADS), 0, s, n, T, P, W.abar, W.xbar, W.vbar, W.rbar,
+    hipLaunchKernelGGL(k_adj_accel, dim3(ns), dim3(ADJ_THREADS), 0, s, n, T, P, W.abar, W.xbar, W.vbar, W.rbar,
                        W.pbar, W.nbar, W.part_a);
-    hipLaunchKernelGGL(k_adj_normals, dim3(nb), dim3(ADJ_THREADS), 0, s, n, T, cn, nH2, rho0, k_gas, W.nbar, W.pbar,
+    hipLaunchKernelGGL(k_adj_normals, dim3(ns), dim3(ADJ_THREADS), 0, s, n, T, cn, nH2, rho0, k_gas, W.nbar, W.pbar,
                        W.xbar, W.rbar, W.part_b);
-    hipLaunchKernelGGL(k_adj_density, dim3(nb), dim3(ADJ_THREADS), 0, s, n, T, c_rho, h2, fix, W.rbar, W.xbar,
+    hipLaunchKernelGGL(k_adj_density, dim3(ns), dim3(ADJ_THREADS), 0, s, n, T, c_rho, h2, fix, W.rbar, W.xbar,
                        W.vbar, W.vhbar, gx_out, gv_out, gvh_out, capped);
-    hipLaunchKernelGGL(k_adj_params, dim3(1), dim3(64), 0, s, nb, W.part_a, W.part_b, params);
+    hipLaunchKernelGGL(k_adj_params, dim3(1), dim3(64), 0, s, ns, W.part_a, W.part_b, params);
 }
 
 void launch_adjoint_ids(int n, const int* perm, const int* ids_in, int* ids_out, hipStream_t s) {

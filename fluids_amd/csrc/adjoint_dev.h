@@ -25,6 +25,8 @@ struct AdjWork {
 };
 
 int adj_blocks(int n);
+// workgroups of the adjoint kernels that split each particle's list over lanes
+int adj_split_blocks(int n);
 // Backward pass of one WCSPH step: (gx, gv, gvh) are the adjoints of the
 // step's outputs (stored order after the step = sorted order of the step),
 // (gx_out, gv_out, gvh_out) those of its inputs (stored order before the

@@ -585,12 +585,152 @@ __global__ __launch_bounds__(LISTS_THREADS, 8) void k_lists_density(
                              pressure, prho, st, stage + threadIdx.x);
 }
 
+// ------------------------------------------- lists + density, split walk
+// A scene too small to fill the GPU with one thread per particle leaves
+// most SIMDs idle while each thread walks its 27 cells one after the other.
+// There each particle's walk is split over SPLIT consecutive lanes: lane q
+// walks cells [q * SPLIT_CELLS, ...) of CELL_NEIGH_OFFSET order with the
+// same tests as walk_cells and keeps, in LDS, its first LIST_CAP hits (index
+// and r^2), its hit count per cell and its edge reads per cell. The
+// particle's first lane then concatenates the parts in cell order up to the
+// cap, adds the density terms in that order, and counts the edge reads the
+// single walk makes (cell 0's, and cell o + 1's while fewer than LIST_CAP
+// hits precede cell o). Lists, densities and counters equal
+// k_lists_density's; the boundary walk is the first lane's.
+constexpr int SPLIT = 4;
+constexpr int SPLIT_CELLS = 7;  // cells per part; the last part has 6
+constexpr int SPLIT_THREADS = 64;
+constexpr int SPLIT_MAX_N = 1 << 17;  // the split walk serves scenes up to this many particles
+
+template <bool WCSPH>
+__global__ __launch_bounds__(SPLIT_THREADS) void k_lists_density_split(
+    int n, const float4* __restrict__ pos_s, const int2* __restrict__ fR, int nb, const float4* __restrict__ bpos_s,
+    const float* __restrict__ bphi_s, const int2* __restrict__ bR, const uint8_t* __restrict__ bnear, GridDev g,
+    DensityParams P, int* __restrict__ nl, int* __restrict__ nc, int* __restrict__ bnl, int* __restrict__ bnc,
+    float* __restrict__ density, float* __restrict__ pressure, float4* __restrict__ prho, DevState* st) {
+    __shared__ int hj[SPLIT_THREADS][LIST_CAP];
+    __shared__ float hr[SPLIT_THREADS][LIST_CAP];
+    __shared__ int hcnt[SPLIT_THREADS][SPLIT_CELLS];
+    __shared__ int hedge[SPLIT_THREADS], hstored[SPLIT_THREADS];
+    const int tid = threadIdx.x;
+    const int t = xcd_block(blockIdx.x, gridDim.x) * SPLIT_THREADS + tid;
+    const int i = t / SPLIT, q = t % SPLIT;
+    const bool valid = i < n;
+    float4 p4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    int cx = 0, cy = 0, cz = 0;
+    if (valid) {
+        p4 = pos_s[i];
+        grid_coords(p4, g, cx, cy, cz);
+        const V3 pi = v3(p4);
+        const __amdgpu_buffer_rsrc_t rp = buffer_view(pos_s, (uint32_t)n * 16u);
+        const int o0 = q * SPLIT_CELLS, o1 = o0 + SPLIT_CELLS < 27 ? o0 + SPLIT_CELLS : 27;
+        int stored = 0, emask = 0;
+        int dx, dy, dz, e = 0;
+        neigh_offset(o0, dx, dy, dz);
+        int2 cur = cell_range(fR, cx + dx, cy + dy, cz + dz, g, e);
+        emask |= e;
+        for (int o = o0; o < o1; ++o) {
+            int2 nxt = make_int2(0, 0);
+            if (o + 1 < o1) {  // fetch the next cell's range while this cell is processed
+                e = 0;
+                neigh_offset(o + 1, dx, dy, dz);
+                nxt = cell_range(fR, cx + dx, cy + dy, cz + dz, g, e);
+                emask |= e << (o + 1 - o0);
+            }
+            const int a = cur.x, b = cur.y < n ? cur.y : n;
+            int cnt = 0;
+            for (int j0 = a; j0 < b; j0 += WALK_BATCH) {
+                float4 c4[WALK_BATCH];
+#pragma unroll
+                for (int u = 0; u < WALK_BATCH; ++u) c4[u] = load4(rp, (uint32_t)j0 * 16u, 16u * u);
+#pragma unroll
+                for (int u = 0; u < WALK_BATCH; ++u) {
+                    V3 r = pi - v3(c4[u]);
+                    float r2 = dot(r, r);
+                    if (j0 + u < b && r2 < g.sqr_support) {
+                        if (stored < LIST_CAP) {
+                            hj[tid][stored] = j0 + u;
+                            hr[tid][stored] = r2;
+                            ++stored;
+                        }
+                        ++cnt;
+                    }
+                }
+            }
+            hcnt[tid][o - o0] = cnt;
+            cur = nxt;
+        }
+        hedge[tid] = emask;
+        hstored[tid] = stored;
+    }
+    __syncthreads();
+    if (!valid || q != 0) return;
+    // the single walk's list, density and edge reads, from the parts in order
+    int len = 0, edge = 0, before = 0;  // before: hits in the cells before cell o
+    float d = 0.0f;
+    for (int p = 0; p < SPLIT && len < LIST_CAP; ++p) {
+        const int lp = tid + p, m = hstored[lp];
+        for (int k = 0; k < m && len < LIST_CAP; ++k) {
+            nl[(size_t)len * n + i] = hj[lp][k];
+            ++len;
+            accumulate_density<WCSPH>(hr[lp][k], d, P, 1.0f, false);
+        }
+    }
+    edge = hedge[tid] & 1;  // cell 0's range is always read
+    for (int o = 0; o < 27 && before < LIST_CAP; ++o) {
+        const int lp = tid + o / SPLIT_CELLS, c = o % SPLIT_CELLS;
+        if (o + 1 < 27) {
+            const int lq = tid + (o + 1) / SPLIT_CELLS;
+            edge += (hedge[lq] >> ((o + 1) % SPLIT_CELLS)) & 1;
+        }
+        before += hcnt[lp][c];
+    }
+    nc[i] = len;
+    const V3 pi = v3(p4);
+    float bd = 0.0f;
+    int blen = 0;
+    if (nb > 0) {
+        bool skip = false;
+        if (interior_cell(cx, cy, cz, g.cps)) {
+            long long bb = box_index(cx, cy, cz, g);
+            skip = bb >= 0 && bnear[bb] == 0;
+        }
+        if (!skip) {
+            const ListSink<SPLIT_THREADS, 0> bl{nullptr, bnl + i, n};
+            blen = walk_cells<WCSPH, true>(pi, cx, cy, cz, bR, bpos_s, bphi_s, nb, g, P, bl, bd, edge);
+        }
+    }
+    bnc[i] = blen;
+    if (WCSPH) {
+        d *= P.w_eval * P.M;
+        d += P.w_eval * bd;
+        density[i] = d;
+        pressure[i] = cl_max(0.0f, P.k_gas * (pown7(d / P.rho0) - 1.0f));
+    } else {
+        d = P.w_eval * ((d * P.M) + bd);
+        density[i] = d;
+    }
+    prho[i] = make_float4(p4.x, p4.y, p4.z, d);
+    if (edge) atomicAdd(&st->edge_reads, edge);
+}
+
 void launch_lists_density(bool wcsph, int n, const float4* pos_s, const int2* fR, int nb, const float4* bpos_s,
                           const float* bphi_s, const int2* bR, const uint8_t* bnear, const GridDev& g,
                           const DensityParams& P, int* nl, int* nc, int* bnl, int* bnc, float* density,
                           float* pressure, float4* prho, DevState* st, hipStream_t s, Profiler* prof) {
     if (n <= 0) return;
     PROF_BEGIN(prof, "lists_density", s);
+    if (n <= SPLIT_MAX_N) {
+        const int sblocks = (int)(((long long)n * SPLIT + SPLIT_THREADS - 1) / SPLIT_THREADS);
+        if (wcsph)
+            hipLaunchKernelGGL(k_lists_density_split<true>, dim3(sblocks), dim3(SPLIT_THREADS), 0, s, n, pos_s, fR,
+                               nb, bpos_s, bphi_s, bR, bnear, g, P, nl, nc, bnl, bnc, density, pressure, prho, st);
+        else
+            hipLaunchKernelGGL(k_lists_density_split<false>, dim3(sblocks), dim3(SPLIT_THREADS), 0, s, n, pos_s, fR,
+                               nb, bpos_s, bphi_s, bR, bnear, g, P, nl, nc, bnl, bnc, density, pressure, prho, st);
+        PROF_END(prof, s);
+        return;
+    }
     const int blocks = (n + LISTS_THREADS - 1) / LISTS_THREADS;
     if (wcsph)
         hipLaunchKernelGGL(k_lists_density<true>, dim3(blocks), dim3(LISTS_THREADS), 0, s, n, pos_s, fR, nb, bpos_s, bphi_s, bR,
