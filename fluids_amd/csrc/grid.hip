@@ -54,9 +54,14 @@ __global__ __launch_bounds__(256) void k_hash(int n, const float4* __restrict__ 
 // lanes sharing a digit are the AND of 8 bit-ballots), then orders waves
 // through LDS, so the output keeps input order within equal digits.
 constexpr int SORT_THREADS = 256;
-constexpr int SORT_ITEMS = 8;
-constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;
+constexpr int SORT_ITEMS = 8;  // keys per thread of a tile (2048-key tiles)
+// Up to SMALL_SORT_N keys a tile holds one key per thread: a small scene then
+// spreads each pass over more workgroups with one round each, instead of a
+// few workgroups running their rounds one after another.
+constexpr int SMALL_SORT_N = 1 << 16;
 constexpr int RADIX_MAX = 1024;
+
+static int sort_items(int n) { return n <= SMALL_SORT_N ? 1 : SORT_ITEMS; }
 
 // Inclusive scan of one value per thread over a 256-thread workgroup.
 __device__ __forceinline__ uint32_t block_incl_scan_256(uint32_t v, uint32_t* wtot) {
@@ -82,7 +87,7 @@ __device__ __forceinline__ uint32_t radix_digit(uint32_t key, int val, int shift
     return (key >> shift) & ((1u << BITS) - 1u);
 }
 
-template <int BITS, bool SID>
+template <int BITS, bool SID, int ITEMS>
 __global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint32_t* __restrict__ keys,
                                                              const int* __restrict__ vin, int n, int shift,
                                                              int ntiles, uint32_t* __restrict__ hist,
@@ -92,9 +97,9 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint32_t* __r
     const int t = threadIdx.x;
     for (int d = t; d < RADIX; d += SORT_THREADS) h[d] = 0;
     __syncthreads();
-    int base = blockIdx.x * SORT_TILE;
+    int base = blockIdx.x * (SORT_THREADS * ITEMS);
 #pragma unroll 4
-    for (int k = 0; k < SORT_ITEMS; ++k) {
+    for (int k = 0; k < ITEMS; ++k) {
         int idx = base + k * SORT_THREADS + t;
         if (idx < n) {
             int val = SID ? (vin ? vin[idx] : idx) : 0;
@@ -124,7 +129,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_rowscan(uint32_t* __rest
     if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
-template <int BITS, bool SID>
+template <int BITS, bool SID, int ITEMS>
 __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint32_t* __restrict__ kin,
                                                                 const int* __restrict__ vin,
                                                                 uint32_t* __restrict__ kout, int* __restrict__ vout,
@@ -156,8 +161,8 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint32_t* 
     }
     __syncthreads();
     const uint64_t lt_mask = (1ull << lane) - 1ull;
-    const int base = blockIdx.x * SORT_TILE;
-    for (int k = 0; k < SORT_ITEMS; ++k) {
+    const int base = blockIdx.x * (SORT_THREADS * ITEMS);
+    for (int k = 0; k < ITEMS; ++k) {
         int idx = base + k * SORT_THREADS + t;
         bool valid = idx < n;
         uint32_t key = valid ? kin[idx] : 0u;
@@ -215,26 +220,41 @@ int sort_passes(int key_bits) {
     return p;
 }
 
+static int sort_tiles(int n) {
+    const int tile = SORT_THREADS * sort_items(n);
+    return (n + tile - 1) / tile;
+}
+
 size_t sort_hist_words(int n) {
-    int ntiles = (n + SORT_TILE - 1) / SORT_TILE;
+    int ntiles = sort_tiles(n);
     return (size_t)RADIX_MAX * (ntiles > 0 ? ntiles : 1) + RADIX_MAX;
+}
+
+template <int BITS, bool SID, int ITEMS>
+static void radix_pass_items(const uint32_t* kin, const int* vin, uint32_t* kout, int* vout, uint32_t* hist,
+                             uint32_t* totals, int n, int ntiles, int shift, const int* sid, hipStream_t s,
+                             Profiler* prof) {
+    PROF_BEGIN(prof, "radix_hist", s);
+    hipLaunchKernelGGL((k_radix_hist<BITS, SID, ITEMS>), dim3(ntiles), dim3(SORT_THREADS), 0, s, kin, vin, n, shift,
+                       ntiles, hist, sid);
+    PROF_END(prof, s);
+    PROF_BEGIN(prof, "radix_rowscan", s);
+    hipLaunchKernelGGL(k_radix_rowscan, dim3(1 << BITS), dim3(SORT_THREADS), 0, s, hist, ntiles, totals);
+    PROF_END(prof, s);
+    PROF_BEGIN(prof, "radix_scatter", s);
+    hipLaunchKernelGGL((k_radix_scatter<BITS, SID, ITEMS>), dim3(ntiles), dim3(SORT_THREADS), 0, s, kin, vin, kout,
+                       vout, n, shift, ntiles, hist, totals, sid);
+    PROF_END(prof, s);
 }
 
 template <int BITS, bool SID>
 static void radix_pass(const uint32_t* kin, const int* vin, uint32_t* kout, int* vout, uint32_t* hist,
                        uint32_t* totals, int n, int ntiles, int shift, const int* sid, hipStream_t s,
                        Profiler* prof) {
-    PROF_BEGIN(prof, "radix_hist", s);
-    hipLaunchKernelGGL((k_radix_hist<BITS, SID>), dim3(ntiles), dim3(SORT_THREADS), 0, s, kin, vin, n, shift, ntiles,
-                       hist, sid);
-    PROF_END(prof, s);
-    PROF_BEGIN(prof, "radix_rowscan", s);
-    hipLaunchKernelGGL(k_radix_rowscan, dim3(1 << BITS), dim3(SORT_THREADS), 0, s, hist, ntiles, totals);
-    PROF_END(prof, s);
-    PROF_BEGIN(prof, "radix_scatter", s);
-    hipLaunchKernelGGL((k_radix_scatter<BITS, SID>), dim3(ntiles), dim3(SORT_THREADS), 0, s, kin, vin, kout, vout, n,
-                       shift, ntiles, hist, totals, sid);
-    PROF_END(prof, s);
+    if (sort_items(n) == 1)
+        radix_pass_items<BITS, SID, 1>(kin, vin, kout, vout, hist, totals, n, ntiles, shift, sid, s, prof);
+    else
+        radix_pass_items<BITS, SID, SORT_ITEMS>(kin, vin, kout, vout, hist, totals, n, ntiles, shift, sid, s, prof);
 }
 
 int sort_passes(int key_bits, int sid_bits) { return sort_passes(key_bits) + (sid_bits > 0 ? 1 : 0); }
@@ -246,7 +266,7 @@ void launch_radix_sort(uint32_t* keys_a, uint32_t* keys_b, int* vals_a, int* val
     const int bits = digit_bits(key_bits, passes);
     if (sid && sid_bits > 10) throw std::invalid_argument("radix sort: more than 1024 segments");
     const int total = passes + (sid ? 1 : 0);
-    int ntiles = (n + SORT_TILE - 1) / SORT_TILE;
+    const int ntiles = sort_tiles(n);
     uint32_t* totals = hist + (size_t)RADIX_MAX * ntiles;
     // The caller hashed into keys_b when the pass count is odd, keys_a when
     // even; either way the result lands in (keys_a, vals_a).
