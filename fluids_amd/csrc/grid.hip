@@ -436,7 +436,13 @@ __device__ __forceinline__ void accumulate_density(float r2, float& d, const Den
     d += weighted ? t * w : t;
 }
 
-constexpr int WALK_BATCH = 8;
+// Candidates per batch. A cell holds about 8 particles at h = 2 x spacing; a
+// batch runs for the whole wave while any lane has candidates left, so when
+// one of the wave's cells holds 9-12, batches of 4 waste fewer slots than
+// batches of 8 (1M throughput scene: walk 215 -> 202 us; 3 and 6 are slower).
+constexpr int WALK_BATCH = 4;
+// The split walk of small scenes (k_lists_density_split) keeps batches of 8.
+constexpr int SPLIT_BATCH = 8;
 constexpr int LISTS_THREADS = 64;
 // List slots kept in LDS until the walk ends; slots past it are stored
 // straight to the list. Lanes of a wave find neighbours at different times,
@@ -659,12 +665,12 @@ __global__ __launch_bounds__(SPLIT_THREADS) void k_lists_density_split(
             }
             const int a = cur.x, b = cur.y < n ? cur.y : n;
             int cnt = 0;
-            for (int j0 = a; j0 < b; j0 += WALK_BATCH) {
-                float4 c4[WALK_BATCH];
+            for (int j0 = a; j0 < b; j0 += SPLIT_BATCH) {
+                float4 c4[SPLIT_BATCH];
 #pragma unroll
-                for (int u = 0; u < WALK_BATCH; ++u) c4[u] = load4(rp, (uint32_t)j0 * 16u, 16u * u);
+                for (int u = 0; u < SPLIT_BATCH; ++u) c4[u] = load4(rp, (uint32_t)j0 * 16u, 16u * u);
 #pragma unroll
-                for (int u = 0; u < WALK_BATCH; ++u) {
+                for (int u = 0; u < SPLIT_BATCH; ++u) {
                     V3 r = pi - v3(c4[u]);
                     float r2 = dot(r, r);
                     if (j0 + u < b && r2 < g.sqr_support) {
