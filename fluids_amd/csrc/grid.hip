@@ -824,6 +824,7 @@ __global__ void k_step_begin(DevState* st) {
         st->out_of_grid = 0;
         st->edge_reads = 0;
         st->nonfinite = 0;
+        st->bar_arrive = 0u;
     }
 }
 

@@ -128,6 +128,13 @@ void launch_pressure_force(int n, int iter, const float4* pc, const float4* vel_
                            const int* nl, const int* nc, const int* bnl, const int* bnc, const float4* bpos_s,
                            const float* bphi_s, int has_b, const PciParams& P, float4* f_press, float4* pos_pred,
                            float4* pos_out, float4* vel_out, DevState* st, hipStream_t s, Profiler* prof);
+// Corrections first .. P.max_iter in one launch; returns at once when the
+// loop already ended.
+void launch_correction_tail(int n, int first, const float4* prho, const float4* vel_s, const float4* f_other,
+                            const int* nl, const int* nc, const int* bnl, const int* bnc, const float4* bpos_s,
+                            const float* bphi_s, int has_b, const PciParams& P, float* err, float* pressure,
+                            float4* pc, float4* f_press, float4* pos_pred, float4* pos_out, float4* vel_out,
+                            DevState* st, hipStream_t s, Profiler* prof);
 void launch_pci_integrate_only(int n, const float4* pos_s, const float4* vel_s, const float4* f_other,
                                const PciParams& P, float4* pos_out, float4* vel_out, hipStream_t s,
                                Profiler* prof);

@@ -15,6 +15,7 @@
 // every thread of pressure_force from the reduced maximum, so the loop runs
 // without a host round trip; corrections launched past the exit return at once.
 #include "kernels.h"
+#include "solver.h"
 #include "sph_device.h"
 
 namespace sph {
@@ -120,17 +121,16 @@ __global__ __launch_bounds__(256) void k_forces_predict(int n, const float4* __r
 // update_pressure also packs (x, y, z, p/rho^2) of every particle for the
 // pressure force that follows: p_j/SQR(rho_j) of pcisph.cl:482 and
 // C = p_i/SQR(rho_i) of pcisph.cl:452 are this same value.
-__global__ __launch_bounds__(256) void k_update_pressure(int n, int iter, const float4* __restrict__ pos_pred,
-                                                         const float4* __restrict__ prho,
-                                                         const int* __restrict__ nl, const int* __restrict__ nc,
-                                                         const int* __restrict__ bnl, const int* __restrict__ bnc,
-                                                         const float4* __restrict__ bpos_s,
-                                                         const float* __restrict__ bphi_s, int has_b, PciParams P,
-                                                         float* __restrict__ err, float* __restrict__ pressure,
-                                                         float4* __restrict__ pc, DevState* st) {
+// One correction's update_pressure over the whole grid (grid-stride loop).
+__device__ __forceinline__ void update_pressure_phase(int n, int iter, const float4* __restrict__ pos_pred,
+                                                      const float4* __restrict__ prho,
+                                                      const int* __restrict__ nl, const int* __restrict__ nc,
+                                                      const int* __restrict__ bnl, const int* __restrict__ bnc,
+                                                      const float4* __restrict__ bpos_s,
+                                                      const float* __restrict__ bphi_s, int has_b, const PciParams& P,
+                                                      float* __restrict__ err, float* __restrict__ pressure,
+                                                      float4* __restrict__ pc, DevState* st) {
     __shared__ uint32_t wmax[4];
-    int done = st->done_iter;
-    if (done != 0 && done < iter) return;
     const int stride = gridDim.x * blockDim.x;
     float dvmax = 0.0f;
     for (int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -200,15 +200,30 @@ __global__ __launch_bounds__(256) void k_update_pressure(int n, int iter, const 
     }
 }
 
-__global__ __launch_bounds__(256) void k_pressure_force(
+__global__ __launch_bounds__(256) void k_update_pressure(int n, int iter, const float4* __restrict__ pos_pred,
+                                                         const float4* __restrict__ prho,
+                                                         const int* __restrict__ nl, const int* __restrict__ nc,
+                                                         const int* __restrict__ bnl, const int* __restrict__ bnc,
+                                                         const float4* __restrict__ bpos_s,
+                                                         const float* __restrict__ bphi_s, int has_b, PciParams P,
+                                                         float* __restrict__ err, float* __restrict__ pressure,
+                                                         float4* __restrict__ pc, DevState* st) {
+    int done = st->done_iter;
+    if (done != 0 && done < iter) return;
+    update_pressure_phase(n, iter, pos_pred, prho, nl, nc, bnl, bnc, bpos_s, bphi_s, has_b, P, err, pressure, pc, st);
+}
+
+// One correction's pressure force and prediction (or final integration)
+// over the whole grid. Reads the maximum error that every block of the
+// update_pressure phase has merged; block 0 records the end of the loop.
+__device__ __forceinline__ void pressure_force_phase(
     int n, int iter, const float4* __restrict__ pc, const float4* __restrict__ vel_s,
     const float4* __restrict__ f_other, const int* __restrict__ nl, const int* __restrict__ nc,
     const int* __restrict__ bnl, const int* __restrict__ bnc, const float4* __restrict__ bpos_s,
-    const float* __restrict__ bphi_s, int has_b, PciParams P, float4* __restrict__ f_press,
+    const float* __restrict__ bphi_s, int has_b, const PciParams& P, float4* __restrict__ f_press,
     float4* __restrict__ pos_pred, float4* __restrict__ pos_out, float4* __restrict__ vel_out, DevState* st) {
-    int done = st->done_iter;
-    if (done != 0 && done < iter) return;
-    float mx = __uint_as_float(st->maxerr[iter & (MAXERR_SLOTS - 1)]);
+    float mx = __uint_as_float(
+        __hip_atomic_load(&st->maxerr[iter & (MAXERR_SLOTS - 1)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     bool last = iter >= P.max_iter || (iter >= P.min_iter && ((mx - P.rho0_host) / P.rho0_host < P.eta));
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (last) {
@@ -271,6 +286,45 @@ __global__ __launch_bounds__(256) void k_pressure_force(
 }
 }
 
+__global__ __launch_bounds__(256) void k_pressure_force(
+    int n, int iter, const float4* __restrict__ pc, const float4* __restrict__ vel_s,
+    const float4* __restrict__ f_other, const int* __restrict__ nl, const int* __restrict__ nc,
+    const int* __restrict__ bnl, const int* __restrict__ bnc, const float4* __restrict__ bpos_s,
+    const float* __restrict__ bphi_s, int has_b, PciParams P, float4* __restrict__ f_press,
+    float4* __restrict__ pos_pred, float4* __restrict__ pos_out, float4* __restrict__ vel_out, DevState* st) {
+    int done = st->done_iter;
+    if (done != 0 && done < iter) return;
+    pressure_force_phase(n, iter, pc, vel_s, f_other, nl, nc, bnl, bnc, bpos_s, bphi_s, has_b, P, f_press, pos_pred,
+                         pos_out, vel_out, st);
+}
+
+// Corrections min_iter + 1 .. max_iter in one launch. They run only when the
+// exit test failed after min_iter corrections, which the reference's
+// parameters make rare, so the usual cost is one launch that returns at once
+// (instead of two per possible correction). When they do run, every block
+// of the grid takes part in each phase and the phases are separated by grid
+// barriers; the launcher keeps the grid within what the GPU holds resident at
+// once. The exit decision is read after a barrier, so all blocks leave the
+// loop at the same correction; a barrier that times out ends the launch and
+// is reported by the step statistics.
+__global__ __launch_bounds__(256) void k_correction_tail(
+    int n, int first, const float4* __restrict__ prho, const float4* __restrict__ vel_s,
+    const float4* __restrict__ f_other, const int* __restrict__ nl, const int* __restrict__ nc,
+    const int* __restrict__ bnl, const int* __restrict__ bnc, const float4* __restrict__ bpos_s,
+    const float* __restrict__ bphi_s, int has_b, PciParams P, float* err, float* pressure, float4* pc,
+    float4* f_press, float4* pos_pred, float4* pos_out, float4* vel_out, DevState* st) {
+    for (int iter = first; iter <= P.max_iter; ++iter) {
+        const int done = __hip_atomic_load(&st->done_iter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (done != 0 && done < iter) return;
+        update_pressure_phase(n, iter, pos_pred, prho, nl, nc, bnl, bnc, bpos_s, bphi_s, has_b, P, err, pressure, pc,
+                              st);
+        if (!grid_barrier(st)) return;
+        pressure_force_phase(n, iter, pc, vel_s, f_other, nl, nc, bnl, bnc, bpos_s, bphi_s, has_b, P, f_press,
+                             pos_pred, pos_out, vel_out, st);
+        if (!grid_barrier(st)) return;
+    }
+}
+
 // Final integration with a zero pressure force, for a loop of zero corrections.
 __global__ __launch_bounds__(256) void k_integrate_only(int n, const float4* __restrict__ pos_s,
                                                         const float4* __restrict__ vel_s,
@@ -285,15 +339,27 @@ __global__ __launch_bounds__(256) void k_integrate_only(int n, const float4* __r
     vel_out[i] = f4(v, 0.0f);
 }
 
-// Corrections up to min_iter always run and get one thread per particle.
-// Later ones run only when the exit test failed, which the reference's
-// parameters make rare (its test passes unless a density error reaches
-// rho0 * (1 + eta)); they are launched on a small grid that loops over the
-// particles, so a correction that returns at once costs little.
-static int correction_blocks(int n, int iter, const PciParams& P) {
-    int full = (n + 255) / 256;
-    if (iter <= P.min_iter) return full;
-    return full < 512 ? full : 512;
+// Corrections up to min_iter always run and get one thread per particle;
+// later ones run in k_correction_tail.
+static int correction_blocks(int n) { return (n + 255) / 256; }
+
+// Upper bound of the tail grid (two workgroups per CU of a 256-CU device).
+constexpr int TAIL_MAX_BLOCKS = 512;
+
+// Blocks of k_correction_tail that the current device holds resident at once
+// (cached per device), so that its grid barriers can complete.
+static int resident_tail_blocks() {
+    static int cached[64] = {0};
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev >= 0 && dev < 64 && cached[dev] > 0) return cached[dev];
+    int per_cu = 0, cus = 0;
+    hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_correction_tail, 256, 0), "occupancy");
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "device attribute");
+    const int r = per_cu * cus;
+    if (r < 1) throw Error(SPH_ERR_HIP, "k_correction_tail cannot be resident on this device");
+    if (dev >= 0 && dev < 64) cached[dev] = r;
+    return r;
 }
 
 void launch_forces_predict(int n, const float4* prho, const float4* vel_s, const float4* normal, const int* nl,
@@ -311,8 +377,8 @@ void launch_update_pressure(int n, int iter, const float4* pos_pred, const float
                             int has_b, const PciParams& P, float* err, float* pressure, float4* pc, DevState* st,
                             hipStream_t s, Profiler* prof) {
     if (n <= 0) return;
-    PROF_BEGIN(prof, iter > P.min_iter ? "update_pressure_tail" : "update_pressure", s);
-    hipLaunchKernelGGL(k_update_pressure, dim3(correction_blocks(n, iter, P)), dim3(256), 0, s, n, iter, pos_pred, prho, nl, nc,
+    PROF_BEGIN(prof, "update_pressure", s);
+    hipLaunchKernelGGL(k_update_pressure, dim3(correction_blocks(n)), dim3(256), 0, s, n, iter, pos_pred, prho, nl, nc,
                        bnl, bnc, bpos_s, bphi_s, has_b, P, err, pressure, pc, st);
     PROF_END(prof, s);
 }
@@ -322,9 +388,24 @@ void launch_pressure_force(int n, int iter, const float4* pc, const float4* vel_
                            const float* bphi_s, int has_b, const PciParams& P, float4* f_press, float4* pos_pred,
                            float4* pos_out, float4* vel_out, DevState* st, hipStream_t s, Profiler* prof) {
     if (n <= 0) return;
-    PROF_BEGIN(prof, iter > P.min_iter ? "pressure_force_tail" : "pressure_force", s);
-    hipLaunchKernelGGL(k_pressure_force, dim3(correction_blocks(n, iter, P)), dim3(256), 0, s, n, iter, pc, vel_s, f_other, nl, nc,
+    PROF_BEGIN(prof, "pressure_force", s);
+    hipLaunchKernelGGL(k_pressure_force, dim3(correction_blocks(n)), dim3(256), 0, s, n, iter, pc, vel_s, f_other, nl, nc,
                        bnl, bnc, bpos_s, bphi_s, has_b, P, f_press, pos_pred, pos_out, vel_out, st);
+    PROF_END(prof, s);
+}
+
+void launch_correction_tail(int n, int first, const float4* prho, const float4* vel_s, const float4* f_other,
+                            const int* nl, const int* nc, const int* bnl, const int* bnc, const float4* bpos_s,
+                            const float* bphi_s, int has_b, const PciParams& P, float* err, float* pressure,
+                            float4* pc, float4* f_press, float4* pos_pred, float4* pos_out, float4* vel_out,
+                            DevState* st, hipStream_t s, Profiler* prof) {
+    if (n <= 0 || first > P.max_iter) return;
+    int blocks = correction_blocks(n);
+    const int cap = TAIL_MAX_BLOCKS < resident_tail_blocks() ? TAIL_MAX_BLOCKS : resident_tail_blocks();
+    if (blocks > cap) blocks = cap;
+    PROF_BEGIN(prof, "correction_tail", s);
+    hipLaunchKernelGGL(k_correction_tail, dim3(blocks), dim3(256), 0, s, n, first, prho, vel_s, f_other, nl, nc, bnl,
+                       bnc, bpos_s, bphi_s, has_b, P, err, pressure, pc, f_press, pos_pred, pos_out, vel_out, st);
     PROF_END(prof, s);
 }
 

@@ -731,6 +731,8 @@ void SimulationBase::stats(sph_step_stats& o) {
     launch_stats(n_, d_nc.get(), d_bnc.get(), d_pos.get(), d_acc.get(), stream_);
     hip_check(hipMemcpyAsync(acc, d_acc.get(), sizeof acc, hipMemcpyDeviceToHost, stream_), "download");
     hip_check(hipStreamSynchronize(stream_), "sync");
+    if (st.bar_timeouts != 0)
+        throw Error(SPH_ERR_HIP, "PCISPH correction loop: a grid barrier timed out; the step's results are invalid");
     o.iterations = st.iterations;
     o.edge_reads = st.edge_reads;
     o.out_of_grid = st.out_of_grid;
@@ -872,25 +874,22 @@ void PCISPHSimulation::simulate_step(int min_iter, int max_iter) {
     }
     launch_forces_predict(n_, d_prho.get(), d_vel_s.get(), d_normal.get(), d_nl.get(), d_nc.get(), P_,
                           d_f_other.get(), d_pos_pred.get(), s, &prof_);
-    // Corrections are launched in chunks without host round trips; launches
-    // past the device-side exit return immediately. Only very long loops
-    // (the relaxation step's 10000) check the device flag between chunks.
-    const int chunk = 8;
-    for (int i = 1; i <= max_iter; ++i) {
+    // The corrections that always run get one launch per phase; the rest of
+    // the loop, up to max_iter, is one launch that ends at the device-side
+    // exit test, so the step needs no host round trip.
+    const int always = min_iter < max_iter ? min_iter : max_iter;
+    for (int i = 1; i <= always; ++i) {
         launch_update_pressure(n_, i, d_pos_pred.get(), d_prho.get(), d_nl.get(), d_nc.get(), d_bnl.get(),
                                d_bnc.get(), boundary_.d_pos_s.get(), boundary_.d_phi_s.get(), has_b, P_, d_err.get(),
                                d_pressure.get(), d_pc.get(), d_state.get(), s, &prof_);
         launch_pressure_force(n_, i, d_pc.get(), d_vel_s.get(), d_f_other.get(), d_nl.get(), d_nc.get(), d_bnl.get(),
                               d_bnc.get(), boundary_.d_pos_s.get(), boundary_.d_phi_s.get(), has_b, P_,
                               d_f_press.get(), d_pos_pred.get(), d_pos.get(), d_vel.get(), d_state.get(), s, &prof_);
-        if (i % chunk == 0 && i < max_iter) {
-            int done = 0;
-            hip_check(hipMemcpyAsync(&done, &d_state.get()->done_iter, sizeof(int), hipMemcpyDeviceToHost, s),
-                      "download");
-            hip_check(hipStreamSynchronize(s), "sync");
-            if (done) break;
-        }
     }
+    launch_correction_tail(n_, always + 1, d_prho.get(), d_vel_s.get(), d_f_other.get(), d_nl.get(), d_nc.get(),
+                           d_bnl.get(), d_bnc.get(), boundary_.d_pos_s.get(), boundary_.d_phi_s.get(), has_b, P_,
+                           d_err.get(), d_pressure.get(), d_pc.get(), d_f_press.get(), d_pos_pred.get(), d_pos.get(),
+                           d_vel.get(), d_state.get(), s, &prof_);
     boundary_.apply_fluid_forces();
 }
 
