@@ -13,7 +13,9 @@
 //                        predict_vel_n_pos (pcisph.cl:240-284)
 // The exit test (pcisphsimluation.cpp:142-147) is evaluated on the device by
 // every thread of pressure_force from the reduced maximum, so the loop runs
-// without a host round trip; corrections launched past the exit return at once.
+// without a host round trip. The corrections that always run get one launch
+// per phase; the rest of the loop is one launch (correction_tail) that returns
+// at once when the loop has ended.
 #include "kernels.h"
 #include "solver.h"
 #include "sph_device.h"
