@@ -468,18 +468,6 @@ struct ListSink {
     }
 };
 
-// A buffer view of an array of `bytes` bytes (raw, no stride): loads at an
-// offset past the end read zero. Offsets are 32-bit; the launchers keep the
-// particle counts below MAX_SCENE_PARTICLES so that they fit.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_view(const void* p, uint32_t bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
-}
-__device__ __forceinline__ float4 load4(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff) {
-    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, off, soff, 0));
-}
-__device__ __forceinline__ float load1(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
-}
 
 // The hits of one batch of candidates j0 .. j0 + WALK_BATCH - 1, in order.
 // With CHECKS the batch may run past the cell's end (rem candidates left)
@@ -776,13 +764,14 @@ __device__ __forceinline__ void normals_one(int i, int n, const float4* __restri
     V3 acc = {0.0f, 0.0f, 0.0f};
     const int len = nc[i];
     const int* L = nl + i;
+    const __amdgpu_buffer_rsrc_t rp = buffer_view(prho, (uint32_t)n * 16u);
     for (int k0 = 0; k0 < len; k0 += GATHER_BATCH) {
         int jj[GATHER_BATCH];
         float4 q[GATHER_BATCH];
 #pragma unroll
         for (int u = 0; u < GATHER_BATCH; ++u) jj[u] = k0 + u < len ? L[(size_t)(k0 + u) * n] : i;
 #pragma unroll
-        for (int u = 0; u < GATHER_BATCH; ++u) q[u] = prho[jj[u]];
+        for (int u = 0; u < GATHER_BATCH; ++u) q[u] = load4(rp, (uint32_t)jj[u] * 16u, 0u);
 #pragma unroll
         for (int u = 0; u < GATHER_BATCH; ++u) {
             if (k0 + u < len) {

@@ -85,6 +85,9 @@ __global__ __launch_bounds__(256) void k_forces_predict(int n, const float4* __r
     V3 fv = {0.0f, 0.0f, 0.0f}, fcurv = {0.0f, 0.0f, 0.0f}, fcoh = {0.0f, 0.0f, 0.0f};
     const int len = nc[i];
     const int* L = nl + i;
+    const __amdgpu_buffer_rsrc_t rp = buffer_view(prho, (uint32_t)n * 16u);
+    const __amdgpu_buffer_rsrc_t rv = buffer_view(vel_s, (uint32_t)n * 16u);
+    const __amdgpu_buffer_rsrc_t rn = buffer_view(normal, (uint32_t)n * 16u);
     for (int k0 = 0; k0 < len; k0 += FORCE_BATCH) {
         int jj[FORCE_BATCH];
         float4 qp[FORCE_BATCH], qv[FORCE_BATCH], qn[FORCE_BATCH];
@@ -92,9 +95,10 @@ __global__ __launch_bounds__(256) void k_forces_predict(int n, const float4* __r
         for (int u = 0; u < FORCE_BATCH; ++u) jj[u] = k0 + u < len ? L[(size_t)(k0 + u) * n] : i;
 #pragma unroll
         for (int u = 0; u < FORCE_BATCH; ++u) {
-            qp[u] = prho[jj[u]];
-            qv[u] = vel_s[jj[u]];
-            qn[u] = normal[jj[u]];
+            const uint32_t off = (uint32_t)jj[u] * 16u;
+            qp[u] = load4(rp, off, 0u);
+            qv[u] = load3(rv, off, 0u);
+            qn[u] = load3(rn, off, 0u);
         }
 #pragma unroll
         for (int u = 0; u < FORCE_BATCH; ++u) {
@@ -141,12 +145,13 @@ __device__ __forceinline__ void update_pressure_phase(int n, int iter, const flo
         const int len = nc[i];
         const int rows = wave_max_small(len);
         const int dflt = wave_first(i);
+        const __amdgpu_buffer_rsrc_t rpp = buffer_view(pos_pred, (uint32_t)n * 16u);
         int jj[GATHER_BATCH];
         if (rows > 0) list_batch(nl, n, i, len, rows, 0, dflt, jj);
         for (int k0 = 0; k0 < rows; k0 += GATHER_BATCH) {
             float4 q[GATHER_BATCH];
 #pragma unroll
-            for (int u = 0; u < GATHER_BATCH; ++u) q[u] = pos_pred[jj[u]];
+            for (int u = 0; u < GATHER_BATCH; ++u) q[u] = load3(rpp, (uint32_t)jj[u] * 16u, 0u);
             if (k0 + GATHER_BATCH < rows) list_batch(nl, n, i, len, rows, k0 + GATHER_BATCH, dflt, jj);
 #pragma unroll
             for (int u = 0; u < GATHER_BATCH; ++u) {
@@ -242,13 +247,14 @@ __device__ __forceinline__ void pressure_force_phase(
     V3 f = {0.0f, 0.0f, 0.0f}, fb = {0.0f, 0.0f, 0.0f};
     const int len = nc[i];
     const int* L = nl + i;
+    const __amdgpu_buffer_rsrc_t rpc = buffer_view(pc, (uint32_t)n * 16u);
     for (int k0 = 0; k0 < len; k0 += GATHER_BATCH) {
         int jj[GATHER_BATCH];
         float4 q[GATHER_BATCH];
 #pragma unroll
         for (int u = 0; u < GATHER_BATCH; ++u) jj[u] = k0 + u < len ? L[(size_t)(k0 + u) * n] : i;
 #pragma unroll
-        for (int u = 0; u < GATHER_BATCH; ++u) q[u] = pc[jj[u]];
+        for (int u = 0; u < GATHER_BATCH; ++u) q[u] = load4(rpc, (uint32_t)jj[u] * 16u, 0u);
 #pragma unroll
         for (int u = 0; u < GATHER_BATCH; ++u) {
             if (k0 + u < len) {

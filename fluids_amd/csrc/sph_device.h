@@ -29,6 +29,25 @@ constexpr int GATHER_BATCH = 8;  // neighbour loads issued together per thread
 // offsets.
 constexpr int MAX_SCENE_PARTICLES = 1 << 27;
 
+// A buffer view of an array of `bytes` bytes (raw, no stride): loads at an
+// offset past the end read zero. Offsets are 32-bit; the launchers keep the
+// particle counts below MAX_SCENE_PARTICLES so that 16-byte records fit.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_view(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float4 load4(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff) {
+    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, off, soff, 0));
+}
+// The first three words of a 16-byte record (its w is not fetched).
+__device__ __forceinline__ float4 load3(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff) {
+    typedef unsigned int u32x3 __attribute__((ext_vector_type(3)));
+    const u32x3 v = __builtin_amdgcn_raw_buffer_load_b96(r, off, soff, 0);
+    return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), 0.0f);
+}
+__device__ __forceinline__ float load1(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
+}
+
 // Uniform grid restated from kernels/grid.h and fluid/simulation/grid.cpp:61-68.
 // Cell ranges live in a dense box: all cps cells in y (the 5 m grid is
 // periodic and the integrators allow y up to 10 m), [bx0, bx0+bnx) in x and
