@@ -56,6 +56,10 @@ __device__ __forceinline__ void accel_integrate_one(
     V3 pacc = {0.0f, 0.0f, 0.0f}, vacc = {0.0f, 0.0f, 0.0f}, fcurv = {0.0f, 0.0f, 0.0f}, fcoh = {0.0f, 0.0f, 0.0f};
     const int len = nc[i];
     const int* L = nl + i;
+    const __amdgpu_buffer_rsrc_t rp = buffer_view(prho, (uint32_t)n * 16u);
+    const __amdgpu_buffer_rsrc_t rv = buffer_view(vel_s, (uint32_t)n * 16u);
+    const __amdgpu_buffer_rsrc_t rn = buffer_view(normal, (uint32_t)n * 16u);
+    const __amdgpu_buffer_rsrc_t rq = buffer_view(pressure, (uint32_t)n * 4u);
     for (int k0 = 0; k0 < len; k0 += WC_BATCH) {
         int jj[WC_BATCH];
         float4 qp[WC_BATCH], qv[WC_BATCH], qn[WC_BATCH];
@@ -64,10 +68,11 @@ __device__ __forceinline__ void accel_integrate_one(
         for (int u = 0; u < WC_BATCH; ++u) jj[u] = k0 + u < len ? L[(size_t)(k0 + u) * n] : i;
 #pragma unroll
         for (int u = 0; u < WC_BATCH; ++u) {
-            qp[u] = prho[jj[u]];
-            qv[u] = vel_s[jj[u]];
-            qn[u] = normal[jj[u]];
-            qpr[u] = pressure[jj[u]];
+            const uint32_t j = (uint32_t)jj[u];
+            qp[u] = load4(rp, j * 16u, 0u);
+            qv[u] = load3(rv, j * 16u, 0u);
+            qn[u] = load3(rn, j * 16u, 0u);
+            qpr[u] = load1(rq, j * 4u, 0u);
         }
 #pragma unroll
         for (int u = 0; u < WC_BATCH; ++u) {
