@@ -87,6 +87,7 @@ SIGNATURES = {
     "sph_set_profiling_filter": (I, [P, C.c_char_p]),
     "sph_get_kernel_times": (I, [P, P, P, P, I]),
     "sph_reset_kernel_times": (I, [P]),
+    "sph_set_barrier_timeout": (I, [P, C.c_double]),
     "sph_sample_box_volume": (I, [P, P, F, P, I]),
     "sph_sample_fishtank": (I, [F, F, F, F, P, I]),
     "sph_sample_cube": (I, [F, F, P, I]),

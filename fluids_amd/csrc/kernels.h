@@ -73,6 +73,7 @@ struct PciParams {
     float lo_x, hi_x, lo_y, lo_z, hi_z;  // container clamp: (-c)/2 and c/2; y is clamped to [lo_y, 10]
     float rho0_host, eta;                // exit test operands (host floats)
     int min_iter, max_iter;
+    unsigned long long barrier_ticks;    // bound of each grid-barrier wait, device wall-clock ticks
 };
 
 struct WcParams {

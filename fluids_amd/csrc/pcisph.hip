@@ -326,10 +326,10 @@ __global__ __launch_bounds__(256) void k_correction_tail(
         if (done != 0 && done < iter) return;
         update_pressure_phase(n, iter, pos_pred, prho, nl, nc, bnl, bnc, bpos_s, bphi_s, has_b, P, err, pressure, pc,
                               st);
-        if (!grid_barrier(st)) return;
+        if (!grid_barrier(st, P.barrier_ticks)) return;
         pressure_force_phase(n, iter, pc, vel_s, f_other, nl, nc, bnl, bnc, bpos_s, bphi_s, has_b, P, f_press,
                              pos_pred, pos_out, vel_out, st);
-        if (!grid_barrier(st)) return;
+        if (!grid_barrier(st, P.barrier_ticks)) return;
     }
 }
 

@@ -749,6 +749,19 @@ void SimulationBase::stats(sph_step_stats& o) {
 }
 
 // ----------------------------------------------------------------- PCISPH
+// Frequency of the device wall clock that kernels read (cached per device).
+static double device_wall_clock_hz() {
+    static int cached_khz[64] = {0};
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev >= 0 && dev < 64 && cached_khz[dev] > 0) return 1000.0 * cached_khz[dev];
+    int khz = 0;
+    hip_check(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev), "device attribute");
+    if (khz <= 0) throw Error(SPH_ERR_HIP, "device reports no wall-clock rate");
+    if (dev >= 0 && dev < 64) cached_khz[dev] = khz;
+    return 1000.0 * khz;
+}
+
 PCISPHSimulation::PCISPHSimulation(const Settings& s) : SimulationBase(s) { initialize_params(); }
 
 // _initialize_params + _deduce_density_scale_factor
@@ -827,6 +840,9 @@ void PCISPHSimulation::fill_params() {
     P.eta = s_.pcisph_error_ratio;
     P.min_iter = 3;
     P.max_iter = s_.pcisph_max_iterations;
+    P.barrier_ticks = barrier_timeout_s_ > 0.0
+                          ? (unsigned long long)(barrier_timeout_s_ * device_wall_clock_hz())
+                          : 0ull;
 }
 
 // relaxation step and velocity reset (pcisphsimluation.cpp:548-550)

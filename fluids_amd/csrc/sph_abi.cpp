@@ -345,6 +345,15 @@ int sph_set_profiling_filter(sph_handle* h, const char* kernel) {
     });
 }
 
+int sph_set_barrier_timeout(sph_handle* h, double seconds) {
+    return guard([&] {
+        auto* p = dynamic_cast<PCISPHSimulation*>(&sim_of(h));
+        if (!p) throw Error(SPH_ERR_STATE, "barrier timeout applies to PCISPH handles only");
+        p->set_barrier_timeout(seconds);
+        return SPH_OK;
+    });
+}
+
 int sph_get_kernel_times(sph_handle* h, char* names, int* launches, double* total_ms, int cap) {
     return guard([&] {
         auto& p = sim_of(h).profiler();

@@ -238,11 +238,10 @@ __device__ __forceinline__ void list_batch(const int* __restrict__ list, size_t 
 // Thread 0 of each workgroup makes the workgroup's writes visible device-wide
 // (release fence), counts its arrival, and waits until the last arrival
 // advances the round; the acquire fence then makes every other workgroup's
-// writes visible. The wait is bounded by BARRIER_TIMEOUT ticks of the 100 MHz
-// device clock: a workgroup that gives up counts a timeout and returns false,
-// and its caller leaves the kernel, so no wave waits forever.
-constexpr unsigned long long BARRIER_TIMEOUT = 500000000ull;  // 5 s
-__device__ __forceinline__ bool grid_barrier(DevState* st) {
+// writes visible. The wait is bounded by `timeout` ticks of the device wall
+// clock: a workgroup that gives up counts a timeout and returns false, and
+// its caller leaves the kernel, so no wave waits forever.
+__device__ __forceinline__ bool grid_barrier(DevState* st, unsigned long long timeout) {
     __shared__ int ok;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -257,7 +256,7 @@ __device__ __forceinline__ bool grid_barrier(DevState* st) {
         } else {
             const unsigned long long t0 = wall_clock64();
             while (__hip_atomic_load(&st->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-                if (wall_clock64() - t0 > BARRIER_TIMEOUT) {
+                if (wall_clock64() - t0 > timeout) {
                     __hip_atomic_fetch_add(&st->bar_timeouts, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     good = 0;
                     break;

@@ -241,6 +241,17 @@ int sph_set_profiling_filter(sph_handle* h, const char* kernel);
 int sph_get_kernel_times(sph_handle* h, char* names, int* launches, double* total_ms, int cap);
 int sph_reset_kernel_times(sph_handle* h);
 
+/* ----------------------------------------------------------- diagnostics */
+
+/* Bound, in seconds, of each grid-barrier wait in a PCISPH step's correction
+ * loop (default 5). A wait that reaches it ends the loop's launch and makes
+ * the next sph_get_step_stats fail with SPH_ERR_HIP. A value <= 0 gives up at
+ * once; it exists so that tests can force that failure from the host.
+ * SPH_ERR_STATE for a WCSPH handle. No counterpart in the reference, whose
+ * correction loop synchronises with the host after every correction
+ * (pcisphsimluation.cpp:129-148). */
+int sph_set_barrier_timeout(sph_handle* h, double seconds);
+
 /* -------------------------------------------------------------- samplers */
 /* Each returns the sample count; out (xyzw) receives up to cap samples. */
 

@@ -86,3 +86,27 @@ def test_no_fluid_particles():
     sim.synchronize()
     assert sim.positions().shape[0] == 0
     assert sim.step_stats()["nan_positions"] == 0
+
+
+def test_correction_barrier_timeout_is_reported():
+    """Failure handling of the grid barriers in the PCISPH correction loop,
+    forced from the host: with a zero wait bound every workgroup but the
+    last to arrive gives up at its first barrier, the launch ends, and the
+    step statistics raise instead of returning a silently wrong step. The
+    exit test is made to fail (error ratio -1) so that corrections past the
+    third run; 960 particles give the launch four workgroups."""
+    from fluids_amd._lib import SphError
+    base = scenes.small_pcisph()
+    settings = dict(base.settings)
+    settings.update(pcisph_max_iterations=5, pcisph_error_ratio=-1.0)
+    sc = Scene("barrier_timeout", settings, base.container, base.volumes, base.bodies)
+    print("case barrier timeout", sc.n_fluid, flush=True)
+    sim = scenes.build(sc)
+    assert sim.step_stats()["iterations"] == 10000  # the relaxation step ran every correction
+    sim.step(1)
+    assert sim.step_stats()["iterations"] == 5
+    sim.set_barrier_timeout(0.0)
+    sim.step(1)
+    sim.synchronize()
+    with pytest.raises(SphError, match="grid barrier"):
+        sim.step_stats()
