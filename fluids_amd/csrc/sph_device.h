@@ -237,8 +237,9 @@ __device__ __forceinline__ void list_batch(const int* __restrict__ list, size_t 
 // Grid-wide barrier for a launch whose workgroups are all resident at once.
 // Thread 0 of each workgroup makes the workgroup's writes visible device-wide
 // (release fence), counts its arrival, and waits until the last arrival
-// advances the round; the acquire fence then makes every other workgroup's
-// writes visible. The wait is bounded by `timeout` ticks of the device wall
+// advances the round; the wait polls with relaxed loads (an acquiring poll
+// would invalidate the caches on every read) and the fence after it makes
+// every other workgroup's writes visible. The wait is bounded by `timeout` ticks of the device wall
 // clock: a workgroup that gives up counts a timeout and returns false, and
 // its caller leaves the kernel, so no wave waits forever.
 __device__ __forceinline__ bool grid_barrier(DevState* st, unsigned long long timeout) {
@@ -255,7 +256,7 @@ __device__ __forceinline__ bool grid_barrier(DevState* st, unsigned long long ti
             __hip_atomic_fetch_add(&st->bar_gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             const unsigned long long t0 = wall_clock64();
-            while (__hip_atomic_load(&st->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+            while (__hip_atomic_load(&st->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
                 if (wall_clock64() - t0 > timeout) {
                     __hip_atomic_fetch_add(&st->bar_timeouts, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     good = 0;
