@@ -64,7 +64,8 @@ void launch_phi(int n, const float4* p, float H_d, float w_eval, float rho0, flo
 // compute_fluid_force (boundary.cl:61-118): for each sample of a body with
 // mass > 0, the pressure force of the fluid found by walking the fluid cells
 // (cap 50), and its torque about the body's centre. Each workgroup then sums
-// its force and torque in a fixed tree order into one partial (6 floats);
+// its force and torque in a fixed order (wave butterfly shuffles, then the
+// wave sums in wave order) into one partial (6 floats);
 // k_wrench_reduce adds the partials of a body in index order. Both orders are
 // fixed, so the wrench is reproducible run to run (the reference's clsum
 // order is that of Boost.Compute's reduce and is not reproduced).
@@ -81,7 +82,7 @@ __global__ __launch_bounds__(FF_THREADS) void k_fluid_force(int count, int origi
                                                             float cz, float4* __restrict__ b_force,
                                                             float4* __restrict__ b_torque, float* __restrict__ partial,
                                                             DevState* st) {
-    __shared__ float red[6][FF_THREADS];
+    __shared__ float red[6][FF_THREADS / 64];
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     V3 f = {0.0f, 0.0f, 0.0f}, tq = {0.0f, 0.0f, 0.0f};
     if (q < count) {
@@ -114,16 +115,20 @@ __global__ __launch_bounds__(FF_THREADS) void k_fluid_force(int count, int origi
         b_torque[bi] = f4(tq, 0.0f);
         if (edge) atomicAdd(&st->edge_reads, edge);
     }
-    int t = threadIdx.x;
-    red[0][t] = f.x; red[1][t] = f.y; red[2][t] = f.z;
-    red[3][t] = tq.x; red[4][t] = tq.y; red[5][t] = tq.z;
+    // workgroup sums: butterfly shuffles within each wave, then the four wave
+    // sums added in wave order by one thread
+    float v[6] = {f.x, f.y, f.z, tq.x, tq.y, tq.z};
+#pragma unroll
+    for (int c = 0; c < 6; ++c) v[c] = wave_sum(v[c]);
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    if (lane == 0)
+        for (int c = 0; c < 6; ++c) red[c][wave] = v[c];
     __syncthreads();
-    for (int s = FF_THREADS / 2; s > 0; s >>= 1) {
-        if (t < s)
-            for (int c = 0; c < 6; ++c) red[c][t] += red[c][t + s];
-        __syncthreads();
+    if (t < 6) {
+        float s = red[t][0];
+        for (int w = 1; w < FF_THREADS / 64; ++w) s += red[t][w];
+        partial[(size_t)blockIdx.x * 8 + t] = s;
     }
-    if (t < 6) partial[(size_t)blockIdx.x * 8 + t] = red[t][0];
 }
 
 void launch_fluid_force(int count, int origin, const float4* b_unsorted, const float* b_phi, int n,

@@ -272,6 +272,14 @@ __device__ __forceinline__ bool grid_barrier(DevState* st, unsigned long long ti
     return ok != 0;
 }
 
+// Sum over the 64 lanes of a wave by butterfly shuffles (lane 0's value is
+// ((v0 + v32) + (v16 + v48)) + ..., a fixed order). All lanes must be active.
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
 // An index every active lane of the wave holds the same value of.
 __device__ __forceinline__ int wave_first(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
