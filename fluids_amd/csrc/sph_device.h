@@ -9,7 +9,9 @@
 //   clamp(x,lo,hi) = min(max(x,lo),hi)
 //   pown(x,2) = x*x, pown(x,3) = (x*x)*x, pown(x,7) left-to-right products
 //   fast_length(r) = length(r) = sqrt(dot(r,r))
-//   normalize(r) = fast_normalize(r) = r / length(r), and r itself for r == 0
+//   normalize(r) = fast_normalize(r) = r * (1 / length(r)), and r itself for
+//     r == 0 (OpenCL 1.2 defines fast_normalize(p) as p * half_rsqrt(dot(p, p));
+//     the reciprocal square root is taken as two correctly rounded operations)
 // The w component is left out of all arithmetic: positions carry w == 1 and
 // velocities, forces and normals w == 0 on the whole path
 // (boxvolume.cpp:71, boundaryhandler.cpp:132, pcisph.cl:279-280), so every w
@@ -106,8 +108,8 @@ __device__ __forceinline__ float length(V3 a) { return sqrtf(dot(a, a)); }
 __device__ __forceinline__ V3 normalize(V3 a) {
     float d = dot(a, a);
     if (d == 0.0f) return a;
-    float l = sqrtf(d);
-    return {a.x / l, a.y / l, a.z / l};
+    float inv = 1.0f / sqrtf(d);
+    return {a.x * inv, a.y * inv, a.z * inv};
 }
 __device__ __forceinline__ V3 clamp3(V3 a, float lo, float hi) {
     return {cl_clamp(a.x, lo, hi), cl_clamp(a.y, lo, hi), cl_clamp(a.z, lo, hi)};

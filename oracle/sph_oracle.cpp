@@ -11,8 +11,10 @@
 //     -cl-fast-relaxed-math, whose results are implementation-defined; this
 //     file fixes one IEEE interpretation of them);
 //   * fast_length(r) == length(r) == sqrtf(dot(r,r));
-//   * normalize(r) == fast_normalize(r) == r / length(r), and 0 for r == 0
-//     (the OpenCL 1.2 specification of fast_normalize for the zero vector);
+//   * normalize(r) == fast_normalize(r) == r * (1 / length(r)), and 0 for
+//     r == 0 (OpenCL 1.2 defines fast_normalize(p) as p * half_rsqrt(dot(p,p))
+//     and leaves the zero vector unchanged; the reciprocal square root is
+//     taken here as two correctly rounded operations);
 //   * pown(x,2) = x*x, pown(x,3) = (x*x)*x, pown(x,7) = ((((((x*x)*x)*x)*x)*x)*x);
 //   * dot(a,b) = (a.x*b.x + a.y*b.y) + a.z*b.z. The w component drops out:
 //     every position on the path carries w == 1 and every velocity, force and
@@ -86,8 +88,8 @@ template <class T> static inline T length(V3<T> a) { return csqrt(dot(a, a)); }
 template <class T> static inline V3<T> normalize(V3<T> a) {
     T d = dot(a, a);
     if (d == T(0)) return a;
-    T l = csqrt(d);
-    return {a.x / l, a.y / l, a.z / l};
+    T inv = T(1) / csqrt(d);
+    return {a.x * inv, a.y * inv, a.z * inv};
 }
 template <class T> static inline V3<T> cl_clamp3(V3<T> a, T lo, T hi) {
     return {cl_clamp(a.x, lo, hi), cl_clamp(a.y, lo, hi), cl_clamp(a.z, lo, hi)};
@@ -962,17 +964,51 @@ struct Sim {
                 O.b_force[bi] = f;
                 O.b_torque[bi] = {a.y * f.z - a.z * f.y, a.z * f.x - a.x * f.z, a.x * f.y - a.y * f.x};
             }
-            // clsum (Boost.Compute reduce; its summation order is not pinned):
-            // summed here in double, rounded once.
-            double sf[3] = {0, 0, 0}, st[3] = {0, 0, 0};
-            for (int q = 0; q < b.count; ++q) {
-                int bi = b.origin + q;
-                sf[0] += (double)O.b_force[bi].x; sf[1] += (double)O.b_force[bi].y; sf[2] += (double)O.b_force[bi].z;
-                st[0] += (double)O.b_torque[bi].x; st[1] += (double)O.b_torque[bi].y; st[2] += (double)O.b_torque[bi].z;
-            }
-            for (int k = 0; k < 3; ++k) {
-                if constexpr (sizeof(T) == 8) { b.force64[k] = sf[k]; b.torque64[k] = st[k]; }
-                else { b.force[k] = (float)sf[k]; b.torque[k] = (float)st[k]; }
+            // clsum (Boost.Compute reduce; its summation order is not pinned).
+            // fp64: summed in index order. fp32: one pinned order, that of a
+            // 64-lane butterfly: within each run of 64 samples (past the
+            // body's end, +0), level by level, x[l] += x[l ^ off] for
+            // off = 32, 16, ..., 1, keeping x[0]; the four run sums of each
+            // 256 samples are added left to right, and those block sums left
+            // to right starting from +0.
+            if constexpr (sizeof(T) == 8) {
+                double sf[3] = {0, 0, 0}, st[3] = {0, 0, 0};
+                for (int q = 0; q < b.count; ++q) {
+                    int bi = b.origin + q;
+                    sf[0] += (double)O.b_force[bi].x; sf[1] += (double)O.b_force[bi].y; sf[2] += (double)O.b_force[bi].z;
+                    st[0] += (double)O.b_torque[bi].x; st[1] += (double)O.b_torque[bi].y;
+                    st[2] += (double)O.b_torque[bi].z;
+                }
+                for (int k = 0; k < 3; ++k) { b.force64[k] = sf[k]; b.torque64[k] = st[k]; }
+            } else {
+                float acc[6] = {0, 0, 0, 0, 0, 0};
+                for (int blk = 0; blk * 256 < b.count; ++blk) {
+                    float bsum[6] = {0, 0, 0, 0, 0, 0};
+                    for (int w = 0; w < 4; ++w) {
+                        float x[6][64];
+                        for (int l = 0; l < 64; ++l) {
+                            const int q = blk * 256 + w * 64 + l;
+                            const bool in = q < b.count;
+                            const int bi = b.origin + q;
+                            x[0][l] = in ? (float)O.b_force[bi].x : 0.0f;
+                            x[1][l] = in ? (float)O.b_force[bi].y : 0.0f;
+                            x[2][l] = in ? (float)O.b_force[bi].z : 0.0f;
+                            x[3][l] = in ? (float)O.b_torque[bi].x : 0.0f;
+                            x[4][l] = in ? (float)O.b_torque[bi].y : 0.0f;
+                            x[5][l] = in ? (float)O.b_torque[bi].z : 0.0f;
+                        }
+                        for (int c = 0; c < 6; ++c) {
+                            for (int off = 32; off > 0; off >>= 1) {
+                                float y[64];
+                                for (int l = 0; l < 64; ++l) y[l] = x[c][l] + x[c][l ^ off];
+                                for (int l = 0; l < 64; ++l) x[c][l] = y[l];
+                            }
+                            bsum[c] = w == 0 ? x[c][0] : bsum[c] + x[c][0];
+                        }
+                    }
+                    for (int c = 0; c < 6; ++c) acc[c] += bsum[c];
+                }
+                for (int k = 0; k < 3; ++k) { b.force[k] = acc[k]; b.torque[k] = acc[3 + k]; }
             }
         }
     }

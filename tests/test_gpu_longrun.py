@@ -5,10 +5,11 @@
   density error are checked against the oracle's within 0.5 %.
 * The PCISPH double dam break with the 12 kg cube, two-way coupled to the host
   rigid-body world (sph_coupled_step), at a 0.02 m lattice (87 500 particles)
-  over 300 steps: the oracle runs the same loop with its own world. The body
-  wrench is a float sum in a different order on each side, so the two runs
-  are not bit-equal once the cube moves; the cube's trajectory and the
-  fluid's kinetic energy and mean density error must stay within 0.5 %.
+  over 300 steps: the oracle runs the same loop with its own world. Both
+  sides add the body wrench in the same pinned order, so the fluid positions
+  stay bit-equal and the two worlds receive the same wrenches; the cube's
+  trajectory and the fluid's kinetic energy and mean density error are also
+  checked against the 0.5 % contract.
 """
 import os
 
@@ -83,6 +84,8 @@ def test_pcisph_double_dambreak_coupled_cube():
         wo.apply_force_n_torque(ro, f, t)
         wo.step(dt)
         if step % 50 == 0:
+            pg, po = sim.positions(), orc.field("pos")
+            assert np.array_equal(pg, po), f"step {step}: positions differ in {np.sum(pg != po)} values"
             cg, co = wg.state(rg), wo.state(ro)
             moved = max(np.linalg.norm(co["com"] - start), 1e-3)
             dev = float(np.linalg.norm(cg["com"].astype(np.float64) - co["com"]) / moved)
