@@ -389,7 +389,7 @@ void launch_gather_boundary(int n, const int* perm, const uint32_t* keys_s, cons
 // clear has an empty boundary list, so it skips the 27 boundary-range reads.
 __global__ __launch_bounds__(256) void k_mark_boundary(int n, const float4* __restrict__ bpos_s,
                                                        const uint32_t* __restrict__ keys_s, GridDev g,
-                                                       uint8_t* __restrict__ near) {
+                                                       uint32_t* __restrict__ near) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (i > 0 && keys_s[i - 1] == keys_s[i]) return;  // one thread per occupied cell
@@ -401,13 +401,13 @@ __global__ __launch_bounds__(256) void k_mark_boundary(int n, const float4* __re
                 int x = cx - dx, y = cy - dy, z = cz - dz;
                 if (!interior_cell(x, y, z, g.cps)) continue;
                 long long b = box_index(x, y, z, g);
-                if (b >= 0) near[b] = 1;
+                if (b >= 0) atomicOr(&near[b], 1u << neigh_index(dx, dy, dz));
             }
 }
 
-void launch_mark_boundary(int n, const float4* bpos_s, const uint32_t* keys_s, const GridDev& g, uint8_t* near,
-                          size_t near_bytes, hipStream_t s, Profiler* prof) {
-    launch_fill_u32(near, near_bytes / 4, 0u, s);
+void launch_mark_boundary(int n, const float4* bpos_s, const uint32_t* keys_s, const GridDev& g, uint32_t* near,
+                          size_t near_words, hipStream_t s, Profiler* prof) {
+    launch_fill_u32(near, near_words, 0u, s);
     if (n <= 0) return;
     PROF_BEGIN(prof, "mark_boundary", s);
     hipLaunchKernelGGL(k_mark_boundary, dim3((n + 255) / 256), dim3(256), 0, s, n, bpos_s, keys_s, g, near);
@@ -502,18 +502,18 @@ template <bool WCSPH, bool PHI, class Sink>
 __device__ __forceinline__ int walk_cells(V3 pi, int cx, int cy, int cz, const int2* __restrict__ R,
                                           const float4* __restrict__ P4, const float* __restrict__ phi,
                                           int count, const GridDev& g, const DensityParams& P,
-                                          const Sink& list, float& d, int& edge) {
+                                          const Sink& list, float& d, int& edge, uint32_t cells = ALL_CELLS) {
     const __amdgpu_buffer_rsrc_t rp = buffer_view(P4, (uint32_t)count * 16u);
     const __amdgpu_buffer_rsrc_t rw = buffer_view(phi, PHI ? (uint32_t)count * 4u : 0u);
     int len = 0;
-    int2 cur = cell_range(R, cx, cy, cz, g, edge);  // own cell first
+    int2 cur = (cells & 1u) ? cell_range(R, cx, cy, cz, g, edge) : make_int2(0, 0);  // own cell first
     for (int o = 0; o < 27 && len < LIST_CAP; ++o) {
         // fetch the next cell's range while this cell is processed
         int2 nxt = make_int2(0, 0);
         if (o + 1 < 27) {
             int dx, dy, dz;
             neigh_offset(o + 1, dx, dy, dz);
-            nxt = cell_range(R, cx + dx, cy + dy, cz + dz, g, edge);
+            if ((cells >> (o + 1)) & 1u) nxt = cell_range(R, cx + dx, cy + dy, cz + dz, g, edge);
         }
         const int a = cur.x, b = cur.y < count ? cur.y : count;  // ranges never exceed count
         for (int j0 = a; j0 < b && len < LIST_CAP; j0 += WALK_BATCH) {
@@ -542,7 +542,7 @@ template <bool WCSPH, int TPB = LISTS_THREADS, int SLOTS = STAGE_SLOTS>
 __device__ __forceinline__ void lists_density_one(
     int i, int stride, int count, const float4* __restrict__ pos_s, const int2* __restrict__ fR, int nb,
     const float4* __restrict__ bpos_s, const float* __restrict__ bphi_s, const int2* __restrict__ bR,
-    const uint8_t* __restrict__ bnear, const GridDev& g, const DensityParams& P, int* __restrict__ nl,
+    const uint32_t* __restrict__ bnear, const GridDev& g, const DensityParams& P, int* __restrict__ nl,
     int* __restrict__ nc, int* __restrict__ bnl, int* __restrict__ bnc, float* __restrict__ density,
     float* __restrict__ pressure, float4* __restrict__ prho, DevState* st, int* stage_col) {
     float4 p4 = pos_s[i];
@@ -559,14 +559,17 @@ __device__ __forceinline__ void lists_density_one(
     float bd = 0.0f;
     int blen = 0;
     if (nb > 0) {
-        bool skip = false;
+        // an interior cell walks only the neighbour cells that hold boundary
+        // particles (none of its 27 cells is outside the grid, so skipping
+        // the others leaves the edge-read count unchanged)
+        uint32_t cells = ALL_CELLS;
         if (interior_cell(cx, cy, cz, g.cps)) {
             long long b = box_index(cx, cy, cz, g);
-            skip = b >= 0 && bnear[b] == 0;
+            if (b >= 0) cells = bnear[b];
         }
-        if (!skip) {
+        if (cells != 0) {
             const ListSink<TPB, SLOTS> bl{stage_col, bnl + i, stride};
-            blen = walk_cells<WCSPH, true>(pi, cx, cy, cz, bR, bpos_s, bphi_s, nb, g, P, bl, bd, edge);
+            blen = walk_cells<WCSPH, true>(pi, cx, cy, cz, bR, bpos_s, bphi_s, nb, g, P, bl, bd, edge, cells);
             bl.flush(blen);
         }
     }
@@ -589,7 +592,7 @@ __device__ __forceinline__ void lists_density_one(
 template <bool WCSPH>
 __global__ __launch_bounds__(LISTS_THREADS, 8) void k_lists_density(
     int n, const float4* __restrict__ pos_s, const int2* __restrict__ fR, int nb, const float4* __restrict__ bpos_s,
-    const float* __restrict__ bphi_s, const int2* __restrict__ bR, const uint8_t* __restrict__ bnear, GridDev g,
+    const float* __restrict__ bphi_s, const int2* __restrict__ bR, const uint32_t* __restrict__ bnear, GridDev g,
     DensityParams P, int* __restrict__ nl, int* __restrict__ nc, int* __restrict__ bnl, int* __restrict__ bnc,
     float* __restrict__ density, float* __restrict__ pressure, float4* __restrict__ prho, DevState* st) {
     __shared__ int stage[STAGE_SLOTS * LISTS_THREADS];
@@ -619,7 +622,7 @@ constexpr int SPLIT_MAX_N = 1 << 17;  // the split walk serves scenes up to this
 template <bool WCSPH>
 __global__ __launch_bounds__(SPLIT_THREADS) void k_lists_density_split(
     int n, const float4* __restrict__ pos_s, const int2* __restrict__ fR, int nb, const float4* __restrict__ bpos_s,
-    const float* __restrict__ bphi_s, const int2* __restrict__ bR, const uint8_t* __restrict__ bnear, GridDev g,
+    const float* __restrict__ bphi_s, const int2* __restrict__ bR, const uint32_t* __restrict__ bnear, GridDev g,
     DensityParams P, int* __restrict__ nl, int* __restrict__ nc, int* __restrict__ bnl, int* __restrict__ bnc,
     float* __restrict__ density, float* __restrict__ pressure, float4* __restrict__ prho, DevState* st) {
     __shared__ int hj[SPLIT_THREADS][LIST_CAP];
@@ -704,14 +707,14 @@ __global__ __launch_bounds__(SPLIT_THREADS) void k_lists_density_split(
     float bd = 0.0f;
     int blen = 0;
     if (nb > 0) {
-        bool skip = false;
+        uint32_t cells = ALL_CELLS;  // as in lists_density_one
         if (interior_cell(cx, cy, cz, g.cps)) {
             long long bb = box_index(cx, cy, cz, g);
-            skip = bb >= 0 && bnear[bb] == 0;
+            if (bb >= 0) cells = bnear[bb];
         }
-        if (!skip) {
+        if (cells != 0) {
             const ListSink<SPLIT_THREADS, 0> bl{nullptr, bnl + i, n};
-            blen = walk_cells<WCSPH, true>(pi, cx, cy, cz, bR, bpos_s, bphi_s, nb, g, P, bl, bd, edge);
+            blen = walk_cells<WCSPH, true>(pi, cx, cy, cz, bR, bpos_s, bphi_s, nb, g, P, bl, bd, edge, cells);
         }
     }
     bnc[i] = blen;
@@ -729,7 +732,7 @@ __global__ __launch_bounds__(SPLIT_THREADS) void k_lists_density_split(
 }
 
 void launch_lists_density(bool wcsph, int n, const float4* pos_s, const int2* fR, int nb, const float4* bpos_s,
-                          const float* bphi_s, const int2* bR, const uint8_t* bnear, const GridDev& g,
+                          const float* bphi_s, const int2* bR, const uint32_t* bnear, const GridDev& g,
                           const DensityParams& P, int* nl, int* nc, int* bnl, int* bnc, float* density,
                           float* pressure, float4* prho, DevState* st, hipStream_t s, Profiler* prof) {
     if (n <= 0) return;

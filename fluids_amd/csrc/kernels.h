@@ -105,10 +105,10 @@ void launch_gather_fluid(int n, const int* perm, const uint32_t* keys_s, const f
                          hipStream_t s, Profiler* prof);
 void launch_gather_boundary(int n, const int* perm, const uint32_t* keys_s, const float4* pos, const float* phi,
                             float4* pos_s, float* phi_s, int2* R, const GridDev& g, hipStream_t s, Profiler* prof);
-void launch_mark_boundary(int n, const float4* bpos_s, const uint32_t* keys_s, const GridDev& g, uint8_t* near,
-                          size_t near_bytes, hipStream_t s, Profiler* prof);
+void launch_mark_boundary(int n, const float4* bpos_s, const uint32_t* keys_s, const GridDev& g, uint32_t* near,
+                          size_t near_words, hipStream_t s, Profiler* prof);
 void launch_lists_density(bool wcsph, int n, const float4* pos_s, const int2* fR, int nb, const float4* bpos_s,
-                          const float* bphi_s, const int2* bR, const uint8_t* bnear, const GridDev& g,
+                          const float* bphi_s, const int2* bR, const uint32_t* bnear, const GridDev& g,
                           const DensityParams& P, int* nl, int* nc, int* bnl, int* bnc, float* density,
                           float* pressure, float4* prho, DevState* st, hipStream_t s, Profiler* prof);
 void launch_normals(int n, const float4* prho, const int* nl, const int* nc, float H2, float w_grad, float M,

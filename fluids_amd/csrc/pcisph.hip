@@ -269,13 +269,29 @@ __device__ __forceinline__ void pressure_force_phase(
     }
     f = f * ((-sqr(P.M)) * P.press_grad);
     if (has_b) {
-        int blen = bnc[i];
-        for (int k = 0; k < blen; ++k) {
-            int j = bnl[(size_t)k * n + i];
-            V3 r = xi - v3(bpos_s[j]);
-            float l = length(r);
-            V3 gw = (r * (1.0f / l)) * sqr(P.H - l);
-            fb = fb + (bphi_s[j] * (2.0f * C)) * gw;
+        // boundary terms in list order; indices past blen read boundary
+        // particle 0 and are not used
+        const int blen = bnc[i];
+        for (int k0 = 0; k0 < blen; k0 += BOUND_BATCH) {
+            int jj[BOUND_BATCH];
+            float4 q[BOUND_BATCH];
+            float w[BOUND_BATCH];
+#pragma unroll
+            for (int u = 0; u < BOUND_BATCH; ++u) jj[u] = k0 + u < blen ? bnl[(size_t)(k0 + u) * n + i] : 0;
+#pragma unroll
+            for (int u = 0; u < BOUND_BATCH; ++u) {
+                q[u] = bpos_s[jj[u]];
+                w[u] = bphi_s[jj[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < BOUND_BATCH; ++u) {
+                if (k0 + u < blen) {
+                    V3 r = xi - v3(q[u]);
+                    float l = length(r);
+                    V3 gw = (r * (1.0f / l)) * sqr(P.H - l);
+                    fb = fb + (w[u] * (2.0f * C)) * gw;
+                }
+            }
         }
         fb = fb * ((-P.M) * P.press_grad);
     }

@@ -281,7 +281,7 @@ void BoundaryHandler::realloc_ranges() {
     if (d_ranges.size() != cells || !d_ranges.get() || box_version != sim_.grid_.version) {
         d_ranges.release();
         d_ranges.resize(cells);
-        d_near.resize((cells + 3) / 4 * 4);
+        d_near.resize(cells);
         box_version = sim_.grid_.version;
         sorts = 0;
         if (cells) hip_check(hipMemsetAsync(d_ranges.get(), 0, cells * sizeof(int2), sim_.stream_), "hipMemsetAsync");

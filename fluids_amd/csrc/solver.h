@@ -147,7 +147,7 @@ public:
     DeviceArray<uint32_t> d_keys_a, d_keys_b, d_hist;
     DeviceArray<int> d_vals_a, d_vals_b, d_offsets, d_nblocks;
     DeviceArray<int2> d_ranges;
-    DeviceArray<uint8_t> d_near;  // per box cell: a boundary particle lies in its 27-neighbourhood
+    DeviceArray<uint32_t> d_near;  // per box cell: bit o set when neighbour cell o holds boundary particles
     int sorts = 0;
     long long box_version = -1;   // grid box layout d_ranges was built for
     std::vector<Body> bodies_;

@@ -26,6 +26,7 @@ namespace sph {
 constexpr int LIST_CAP = 50;      // NEIGH_LIST_MAX_LENGTH (kernels/common.h:4)
 constexpr int MAXERR_SLOTS = 16;  // ring of per-correction max density errors
 constexpr int GATHER_BATCH = 8;  // neighbour loads issued together per thread
+constexpr int BOUND_BATCH = 4;   // boundary-neighbour loads issued together per thread
 // Largest fluid or boundary particle count of one scene (or one batch): the
 // neighbour walk reads candidates through buffer views with 32-bit byte
 // offsets.
@@ -187,6 +188,9 @@ __device__ __forceinline__ bool interior_cell(int x, int y, int z, int cps) {
     return x >= 1 && y >= 1 && z >= 1 && x <= cps - 2 && y <= cps - 2 && z <= cps - 2;
 }
 
+// Every bit of a 27-cell mask in CELL_NEIGH_OFFSET order.
+constexpr uint32_t ALL_CELLS = (1u << 27) - 1u;
+
 // CELL_NEIGH_OFFSET (kernels/grid.h:15-42): own cell, then the 26 others in
 // lexicographic (dx, dy, dz) order.
 __device__ __forceinline__ void neigh_offset(int o, int& dx, int& dy, int& dz) {
@@ -196,6 +200,11 @@ __device__ __forceinline__ void neigh_offset(int o, int& dx, int& dy, int& dz) {
     dx = k / 9 - 1;
     dy = (k / 3) % 3 - 1;
     dz = k % 3 - 1;
+}
+// Inverse of neigh_offset: the CELL_NEIGH_OFFSET index of (dx, dy, dz).
+__device__ __forceinline__ int neigh_index(int dx, int dy, int dz) {
+    const int k = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
+    return k == 13 ? 0 : (k < 13 ? k + 1 : k);
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs; remap so that each

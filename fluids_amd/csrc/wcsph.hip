@@ -41,17 +41,33 @@ __device__ __forceinline__ void accel_integrate_one(
     float C = pri / sqr(di);
     const float h = P.h;
     V3 bacc = {0.0f, 0.0f, 0.0f};
-    int blen = bnc[i];
-    for (int k = 0; k < blen; ++k) {
-        int j = bnl[(size_t)k * n + i];
-        V3 r = pi - v3(bpos_s[j]);
-        float rn = length(r);
-        V3 t = (-bphi_s[j]) * r;
-        t = t * (P.spiky_grad / rn);
-        t = t * sqr(h - rn);
-        t = t * 2.0f;
-        t = t * C;
-        bacc = bacc + t;
+    // boundary terms in list order; indices past blen read boundary particle
+    // 0 and are not used
+    const int blen = bnc[i];
+    for (int k0 = 0; k0 < blen; k0 += BOUND_BATCH) {
+        int jj[BOUND_BATCH];
+        float4 q[BOUND_BATCH];
+        float w[BOUND_BATCH];
+#pragma unroll
+        for (int u = 0; u < BOUND_BATCH; ++u) jj[u] = k0 + u < blen ? bnl[(size_t)(k0 + u) * n + i] : 0;
+#pragma unroll
+        for (int u = 0; u < BOUND_BATCH; ++u) {
+            q[u] = bpos_s[jj[u]];
+            w[u] = bphi_s[jj[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < BOUND_BATCH; ++u) {
+            if (k0 + u < blen) {
+                V3 r = pi - v3(q[u]);
+                float rn = length(r);
+                V3 t = (-w[u]) * r;
+                t = t * (P.spiky_grad / rn);
+                t = t * sqr(h - rn);
+                t = t * 2.0f;
+                t = t * C;
+                bacc = bacc + t;
+            }
+        }
     }
     V3 pacc = {0.0f, 0.0f, 0.0f}, vacc = {0.0f, 0.0f, 0.0f}, fcurv = {0.0f, 0.0f, 0.0f}, fcoh = {0.0f, 0.0f, 0.0f};
     const int len = nc[i];
