@@ -158,7 +158,7 @@ def run_adjoint(repeats=3):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=200)  # the throughput run is quoted over 200 steps
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--max-iter", type=int, default=None, help="override pcisph_max_iterations")
