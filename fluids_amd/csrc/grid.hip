@@ -449,7 +449,7 @@ constexpr int LISTS_THREADS = 64;
 // so storing each entry when it is found writes a different list row per lane
 // and every row reaches HBM as many partial lines. From LDS the rows go out
 // one at a time, 64 consecutive entries per wave.
-constexpr int STAGE_SLOTS = 24;
+constexpr int STAGE_SLOTS = 28;
 
 template <int TPB, int SLOTS>
 struct ListSink {
