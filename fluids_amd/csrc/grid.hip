@@ -384,9 +384,10 @@ void launch_gather_boundary(int n, const int* perm, const uint32_t* keys_s, cons
     PROF_END(prof, s);
 }
 
-// Marks, for every interior cell, whether a boundary particle lies in one of
-// its 27 neighbour cells. A fluid particle in an interior cell whose mark is
-// clear has an empty boundary list, so it skips the 27 boundary-range reads.
+// Marks, for every interior cell, which of its 27 neighbour cells hold
+// boundary particles: bit o (CELL_NEIGH_OFFSET order) of the cell's mask. A
+// fluid particle in an interior cell reads the boundary ranges of the marked
+// cells only; with no bit set its boundary list is empty and it reads none.
 __global__ __launch_bounds__(256) void k_mark_boundary(int n, const float4* __restrict__ bpos_s,
                                                        const uint32_t* __restrict__ keys_s, GridDev g,
                                                        uint32_t* __restrict__ near) {
@@ -497,7 +498,10 @@ __device__ __forceinline__ void batch_hits(V3 pi, int j0, int rem, const float4 
 // are never used. The distance of every slot is computed unconditionally;
 // the range, cap and radius tests only guard the list entry and the density
 // term, and the range and cap tests are skipped for batches where the whole
-// wave has a full batch left and room for it under the cap.
+// wave has a full batch left and room for it under the cap. Bit o of `cells`
+// (CELL_NEIGH_OFFSET order) selects the cells that are read; the others count
+// as empty without a range read, which callers use only around interior
+// cells, whose 27 cells all lie inside the grid.
 template <bool WCSPH, bool PHI, class Sink>
 __device__ __forceinline__ int walk_cells(V3 pi, int cx, int cy, int cz, const int2* __restrict__ R,
                                           const float4* __restrict__ P4, const float* __restrict__ phi,
