@@ -45,13 +45,14 @@ __global__ __launch_bounds__(256) void k_hash(int n, const float4* __restrict__ 
 }
 
 // ------------------------------------------------------------- radix sort
-// Stable LSD radix sort of (u32 key, i32 value), 8-bit digits. One pass is
+// Stable LSD radix sort of (u32 key, i32 value), 8- to 10-bit digits (the
+// fewest passes over the key's bits). One pass is
 //   hist    per-tile digit counts, stored digit-major hist[d][tile]
 //   rowscan one workgroup per digit: exclusive scan of its row, row total
 //   scatter each tile adds the exclusive prefix of the row totals to its
 //           row entries and writes its keys stably
 // The scatter ranks each 256-key round of a tile with wave64 ballots (the
-// lanes sharing a digit are the AND of 8 bit-ballots), then orders waves
+// lanes sharing a digit are the AND of one ballot per digit bit), then orders waves
 // through LDS, so the output keeps input order within equal digits.
 constexpr int SORT_THREADS = 256;
 constexpr int SORT_ITEMS = 8;  // keys per thread of a tile (2048-key tiles)
