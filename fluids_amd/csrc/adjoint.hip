@@ -358,14 +358,17 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_density(int n, AdjTape T, f
 }
 
 // Sums the per-block parameter partials of one step in a fixed order into
-// double accumulators.
-__global__ void k_adj_params(int nblocks, const float* __restrict__ pa, const float* __restrict__ pb,
-                             double* __restrict__ acc) {
-    if (threadIdx.x >= 3) return;
-    int c = threadIdx.x;
+// double accumulators: one wave per parameter, lane l adding partials
+// l, l + 64, ..., then a butterfly over the lanes.
+constexpr int ADJ_PARAMS = 3;
+__global__ __launch_bounds__(64 * ADJ_PARAMS) void k_adj_params(int nblocks, const float* __restrict__ pa,
+                                                                const float* __restrict__ pb,
+                                                                double* __restrict__ acc) {
+    const int c = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double s = 0.0;
-    for (int b = 0; b < nblocks; ++b) s += (double)pa[3 * b + c] + (double)pb[3 * b + c];
-    acc[c] += s;
+    for (int b = lane; b < nblocks; b += 64) s += (double)pa[3 * b + c] + (double)pb[3 * b + c];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) acc[c] += s;
 }
 
 // ids_out[i] = ids_in[perm[i]]: which particle sits at each stored index.
@@ -394,7 +397,7 @@ void launch_adjoint_step(int n, const AdjTape& T, const WcParams& P, float cn, f
                        W.xbar, W.rbar, W.part_b);
     hipLaunchKernelGGL(k_adj_density, dim3(ns), dim3(ADJ_THREADS), 0, s, n, T, c_rho, h2, fix, W.rbar, W.xbar,
                        W.vbar, W.vhbar, gx_out, gv_out, gvh_out, capped);
-    hipLaunchKernelGGL(k_adj_params, dim3(1), dim3(64), 0, s, ns, W.part_a, W.part_b, params);
+    hipLaunchKernelGGL(k_adj_params, dim3(1), dim3(64 * ADJ_PARAMS), 0, s, ns, W.part_a, W.part_b, params);
 }
 
 void launch_adjoint_ids(int n, const int* perm, const int* ids_in, int* ids_out, hipStream_t s) {
