@@ -119,9 +119,11 @@ void WcAdjoint::run_step(int slot, const WcStepParams& W) {
     fresh_ = false;
     launch_radix_sort(d_keys_a.get(), d_keys_b.get(), d_vals_a.get(), d_vals_b.get(), d_hist.get(), n_,
                       grid_.key_bits, s, nullptr);
-    launch_gather_fluid(n_, d_vals_a.get(), d_keys_a.get(), d_pos.get(), d_vel.get(), d_vh.get(), d_pos_s.get(),
-                        T.vel_s.get(), T.vh_s.get(), d_ranges.get(), grid_.dev, s, nullptr);
-    hip_check(hipMemcpyAsync(T.perm.get(), d_vals_a.get(), n_ * sizeof(int), hipMemcpyDeviceToDevice, s), "copy");
+    // the gather also records the permutation and moves the particle ids
+    // into the new order (into d_ids_b, swapped in below)
+    launch_gather_fluid_tape(n_, d_vals_a.get(), d_keys_a.get(), d_pos.get(), d_vel.get(), d_vh.get(), d_pos_s.get(),
+                             T.vel_s.get(), T.vh_s.get(), d_ranges.get(), grid_.dev, T.perm.get(), d_ids_a.get(),
+                             d_ids_b.get(), s);
     launch_lists_density(true, n_, d_pos_s.get(), d_ranges.get(), 0, nullptr, nullptr, nullptr, nullptr, grid_.dev,
                          W.D, T.nl.get(), T.nc.get(), nullptr, d_bnc.get(), d_density.get(), T.pressure.get(),
                          T.prho.get(), d_state.get(), s, nullptr);
@@ -130,8 +132,7 @@ void WcAdjoint::run_step(int slot, const WcStepParams& W) {
     launch_accel_integrate(n_, T.prho.get(), T.vel_s.get(), T.vh_s.get(), T.pressure.get(), T.normal.get(),
                            T.nl.get(), T.nc.get(), nullptr, d_bnc.get(), nullptr, nullptr, W.P, T.accel.get(),
                            d_pos.get(), d_vel.get(), d_vh.get(), s, nullptr);
-    launch_adjoint_ids(n_, T.perm.get(), d_ids_a.get(), d_ids_b.get(), s);
-    hip_check(hipMemcpyAsync(d_ids_a.get(), d_ids_b.get(), n_ * sizeof(int), hipMemcpyDeviceToDevice, s), "copy");
+    d_ids_a.swap(d_ids_b);
 }
 
 void WcAdjoint::save_checkpoint(int k) {

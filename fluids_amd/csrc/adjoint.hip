@@ -371,15 +371,6 @@ __global__ __launch_bounds__(64 * ADJ_PARAMS) void k_adj_params(int nblocks, con
     if (lane == 0) acc[c] += s;
 }
 
-// ids_out[i] = ids_in[perm[i]]: which particle sits at each stored index.
-__global__ void k_adj_ids(int n, const int* __restrict__ perm, const int* __restrict__ ids_in,
-                          int* __restrict__ ids_out) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int j = perm[i];
-    ids_out[i] = (unsigned)j < (unsigned)n ? ids_in[j] : -1;
-}
-
 int adj_blocks(int n) { return (n + ADJ_THREADS - 1) / ADJ_THREADS; }
 int adj_split_blocks(int n) { return (int)(((long long)n * ADJ_SPLIT + ADJ_THREADS - 1) / ADJ_THREADS); }
 
@@ -398,11 +389,6 @@ void launch_adjoint_step(int n, const AdjTape& T, const WcParams& P, float cn, f
     hipLaunchKernelGGL(k_adj_density, dim3(ns), dim3(ADJ_THREADS), 0, s, n, T, c_rho, h2, fix, W.rbar, W.xbar,
                        W.vbar, W.vhbar, gx_out, gv_out, gvh_out, capped);
     hipLaunchKernelGGL(k_adj_params, dim3(1), dim3(64 * ADJ_PARAMS), 0, s, ns, W.part_a, W.part_b, params);
-}
-
-void launch_adjoint_ids(int n, const int* perm, const int* ids_in, int* ids_out, hipStream_t s) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_adj_ids, dim3(adj_blocks(n)), dim3(ADJ_THREADS), 0, s, n, perm, ids_in, ids_out);
 }
 
 }  // namespace sph

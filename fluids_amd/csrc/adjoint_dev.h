@@ -37,6 +37,5 @@ void launch_adjoint_step(int n, const AdjTape& T, const WcParams& P, float cn, f
                          int fix, float rho0, float k_gas, const float4* gx, const float4* gv, const float4* gvh,
                          AdjWork& W, float4* gx_out, float4* gv_out, float4* gvh_out, double* params, int* capped,
                          hipStream_t s);
-void launch_adjoint_ids(int n, const int* perm, const int* ids_in, int* ids_out, hipStream_t s);
 
 }  // namespace sph

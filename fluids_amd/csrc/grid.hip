@@ -351,6 +351,26 @@ __global__ __launch_bounds__(256) void k_gather_fluid(int n, const int* __restri
     gather_one(i, 0, n, perm, keys_s, pos, vel, vh, pos_s, vel_s, vh_s, R, g);
 }
 
+// The gather of a step the adjoint records: also keeps the permutation and
+// carries the particle ids to the new order (ids_out[i] = ids_in[perm[i]],
+// -1 for an index outside [0, n)).
+__global__ __launch_bounds__(256) void k_gather_fluid_tape(int n, const int* __restrict__ perm,
+                                                           const uint32_t* __restrict__ keys_s,
+                                                           const float4* __restrict__ pos,
+                                                           const float4* __restrict__ vel,
+                                                           const float4* __restrict__ vh, float4* __restrict__ pos_s,
+                                                           float4* __restrict__ vel_s, float4* __restrict__ vh_s,
+                                                           int2* __restrict__ R, GridDev g, int* __restrict__ perm_out,
+                                                           const int* __restrict__ ids_in,
+                                                           int* __restrict__ ids_out) {
+    int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    gather_one(i, 0, n, perm, keys_s, pos, vel, vh, pos_s, vel_s, vh_s, R, g);
+    const int j = perm[i];
+    perm_out[i] = j;
+    ids_out[i] = (unsigned)j < (unsigned)n ? ids_in[j] : -1;
+}
+
 __global__ __launch_bounds__(256) void k_gather_boundary(int n, const int* __restrict__ perm,
                                                          const uint32_t* __restrict__ keys_s,
                                                          const float4* __restrict__ pos, const float* __restrict__ phi,
@@ -374,6 +394,14 @@ void launch_gather_fluid(int n, const int* perm, const uint32_t* keys_s, const f
     hipLaunchKernelGGL(k_gather_fluid, dim3((n + 255) / 256), dim3(256), 0, s, n, perm, keys_s, pos, vel, vh, pos_s,
                        vel_s, vh_s, R, g);
     PROF_END(prof, s);
+}
+
+void launch_gather_fluid_tape(int n, const int* perm, const uint32_t* keys_s, const float4* pos, const float4* vel,
+                              const float4* vh, float4* pos_s, float4* vel_s, float4* vh_s, int2* R, const GridDev& g,
+                              int* perm_out, const int* ids_in, int* ids_out, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_gather_fluid_tape, dim3((n + 255) / 256), dim3(256), 0, s, n, perm, keys_s, pos, vel, vh,
+                       pos_s, vel_s, vh_s, R, g, perm_out, ids_in, ids_out);
 }
 
 void launch_gather_boundary(int n, const int* perm, const uint32_t* keys_s, const float4* pos, const float* phi,

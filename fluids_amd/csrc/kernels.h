@@ -103,6 +103,11 @@ void launch_radix_sort(uint32_t* keys_a, uint32_t* keys_b, int* vals_a, int* val
 void launch_gather_fluid(int n, const int* perm, const uint32_t* keys_s, const float4* pos, const float4* vel,
                          const float4* vh, float4* pos_s, float4* vel_s, float4* vh_s, int2* R, const GridDev& g,
                          hipStream_t s, Profiler* prof);
+// launch_gather_fluid that also writes perm_out = perm and
+// ids_out[i] = ids_in[perm[i]] (the adjoint's tape and particle ids).
+void launch_gather_fluid_tape(int n, const int* perm, const uint32_t* keys_s, const float4* pos, const float4* vel,
+                              const float4* vh, float4* pos_s, float4* vel_s, float4* vh_s, int2* R, const GridDev& g,
+                              int* perm_out, const int* ids_in, int* ids_out, hipStream_t s);
 void launch_gather_boundary(int n, const int* perm, const uint32_t* keys_s, const float4* pos, const float* phi,
                             float4* pos_s, float* phi_s, int2* R, const GridDev& g, hipStream_t s, Profiler* prof);
 void launch_mark_boundary(int n, const float4* bpos_s, const uint32_t* keys_s, const GridDev& g, uint32_t* near,

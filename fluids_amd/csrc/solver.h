@@ -10,6 +10,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "kernels.h"
@@ -56,6 +57,10 @@ public:
     }
     T* get() const { return p_; }
     size_t size() const { return n_; }
+    void swap(DeviceArray& o) {
+        std::swap(p_, o.p_);
+        std::swap(n_, o.n_);
+    }
 
 private:
     T* p_ = nullptr;
