@@ -2,7 +2,8 @@
 
 * 1M particles (the measured workload): two steps on the HIP path and on the
   CPU oracle from the same state, every integer field and every fp32 field
-  compared bit for bit, as in the small parity cases.
+  compared bit for bit, as in the small parity cases; and one step from the
+  state after 300 GPU steps (collapsed boxes, fluid on the floor and walls).
 * 4M particles: the oracle is not run; the GPU's outputs are checked through
   properties that fix them independently of any implementation:
     - the sorted keys are the Morton keys of the sorted positions' cells,
@@ -50,6 +51,34 @@ def test_pcisph_1m_steps_bitexact():
             nbad = int(np.sum(a.view(np.uint32) != b.view(np.uint32)))
             assert nbad == 0, f"step {step}: {f} has {nbad} differing words (max-rel {max_rel(a, b):.2e})"
         print("1M step", step, "bit-exact; mean list length", float(sim.field("ncount").mean()), flush=True)
+
+
+def test_pcisph_1m_late_step_bitexact():
+    """The same scene after 300 GPU steps, when the boxes have collapsed and
+    spread over the floor (longer lists, about one boundary neighbour per
+    particle): one step from the GPU's state on both sides, bit for bit."""
+    sc = scenes.c3_pcisph_throughput(1_000_000)
+    print("case pcisph 1M late", sc.n_fluid, sc.n_boundary, flush=True)
+    sim = scenes.build(sc)
+    orc = build_oracle(sc)
+    sim.step(300)
+    orc.set_state(sim.positions(), sim.velocities())
+    sim.step(1)
+    orc.step(1)
+    st = sim.step_stats()
+    assert st["iterations"] == orc.iterations
+    assert st["boundary_neighbours"] > 300_000, st  # the fluid reaches the walls
+    for f in INT_FIELDS:
+        a, b = sim.field(f), orc.field(f)
+        assert np.array_equal(a, b), f"late: {f} differs at {np.argwhere(a != b)[:5].ravel()}"
+    for f in PCI_FIELDS:
+        a, b = sim.field(f), orc.field(f)
+        if a.ndim == 2:
+            a, b = a[:, :3], b[:, :3]
+        nbad = int(np.sum(a.view(np.uint32) != b.view(np.uint32)))
+        assert nbad == 0, f"late: {f} has {nbad} differing words (max-rel {max_rel(a, b):.2e})"
+    print("1M late step bit-exact; mean list lengths", st["fluid_neighbours"] / sc.n_fluid,
+          st["boundary_neighbours"] / sc.n_fluid, flush=True)
 
 
 def test_pcisph_4m_properties():
