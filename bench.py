@@ -4,12 +4,15 @@ Workload: the 1M-particle PCISPH throughput scene (0.005 m lattice, h =
 0.01 m, Morton re-sort every step, 2x1x1 m tank sampled as one boundary
 layer; fluids_amd.scenes.c3_pcisph_throughput). A step is one full
 simulate() including every PCISPH correction; value = N * K / wall time of
-K timed steps after W warm-up steps, with all state resident in HBM.
+K timed steps (200 by default, as the throughput run is quoted) after W
+warm-up steps, with all state resident in HBM. The cost per step grows over
+the run as the fluid boxes collapse and spread over the floor.
 
 Roofline: algorithmic bytes per particle-update follow the unique-bytes model
 of the reference dataflow (each array a reference kernel reads or writes
 counted once per launch, neighbour gathers as cache hits, list indices int32
-at the measured mean neighbour counts K and Kb). The dominant kernel's
+at the mean neighbour counts K and Kb measured after the timed steps). The
+dominant kernel's
 achieved bandwidth is its algorithmic bytes per launch over its mean launch
 time measured with HIP events on the solver's stream inside the timed region.
 
