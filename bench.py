@@ -18,6 +18,11 @@ time measured with HIP events on the solver's stream inside the timed region.
 
 cpu_baseline: the CPU restatement of the reference kernels (oracle/, fp32,
 same kernel sequence, OpenMP) on the same scene, timed over a few steps.
+
+fast_math: the same scene and step count with SPH_FAST_MATH (approximate
+reciprocal and square root in the PCISPH kernels, as the reference compiles
+them with -cl-fast-relaxed-math). A side figure: the value is the IEEE build,
+whose fields are bit-identical to the oracle's.
 """
 import argparse
 import json
@@ -72,6 +77,36 @@ def run_cpu_baseline(sc, steps, threads):
     o.step(steps)
     dt = time.perf_counter() - t0
     return sc.n_fluid * steps / dt, dt
+
+
+def run_fast_math(n, steps, warmup, max_iter=None):
+    """The same scene with SPH_FAST_MATH (the reference's -cl-fast-relaxed-math
+    arithmetic: approximate reciprocal and square root in the PCISPH kernels),
+    timed like the headline run; a reported side figure, not the value."""
+    from fluids_amd import FAST_MATH, scenes
+    sc = scenes.c3_pcisph_throughput(n)
+    sc.settings["flags"] = FAST_MATH
+    if max_iter is not None:
+        sc.settings["pcisph_max_iterations"] = max_iter
+    sim = scenes.build(sc)
+    sim.step(warmup)
+    sim.reset_kernel_times()
+    sim.set_profiling(True)
+    sim.step(3)
+    sim.synchronize()
+    sim.set_profiling(False)
+    kt = sim.kernel_times()
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+    st = sim.step_stats()
+    return {"value": sc.n_fluid * steps / wall, "unit": "particle-updates/s", "steps": steps,
+            "ms_per_step": 1e3 * wall / steps, "corrections": st["iterations"],
+            "kernels_avg_us": {k: round(1e3 * ms / max(c, 1), 1) for k, (c, ms) in kt.items()},
+            "note": "flags = SPH_FAST_MATH; fields within 1e-5 max-relative of the IEEE build "
+                    "(tests/test_gpu_fast_math.py); the headline value is the IEEE build"}
 
 
 def run_batched(steps, warmup, plain_steps, count=512):
@@ -174,6 +209,7 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=50)
     ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
+    ap.add_argument("--no-fast-math", action="store_true", help="skip the SPH_FAST_MATH side run")
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint timing")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "pmc_pcisph_throughput_1000k.md"),
                     help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary (python -m fluids_amd.profiling pmc) "
@@ -265,6 +301,9 @@ def main():
                "sample": f"{a.cpu_steps} PCISPH steps of the same {N}-particle scene after its relaxation step "
                          f"({dt:.1f} s), fp32 oracle, OpenMP"}
 
+    del sim
+    fast = None if a.no_fast_math else run_fast_math(a.n, a.steps, a.warmup, a.max_iter)
+
     batched = None
     if not a.no_batched:
         batched = run_batched(a.batched_steps, 5, a.plain_steps)
@@ -281,6 +320,7 @@ def main():
         "roofline": roof,
         "step_roofline": {"alg_bytes_per_pu": bpp, "achieved_GBps": bpp * pu_s / 1e9, "frac": bpp * pu_s / HBM_PEAK},
         "cpu_baseline": cpu,
+        "fast_math": fast,
         "batched": batched,
         "adjoint": adjoint,
         "kernels": per,

@@ -793,6 +793,7 @@ void launch_lists_density(bool wcsph, int n, const float4* pos_s, const int2* fR
 
 // ------------------------------------------------------------------ normals
 // compute_normals (normals.cl:23-70); prho holds (x, y, z, density).
+template <bool F = false>
 __device__ __forceinline__ void normals_one(int i, int n, const float4* __restrict__ prho,
                                             const int* __restrict__ nl, const int* __restrict__ nc, float H2,
                                             float w_grad, float M, float H, float4* __restrict__ normal) {
@@ -813,7 +814,7 @@ __device__ __forceinline__ void normals_one(int i, int n, const float4* __restri
             if (k0 + u < len) {
                 V3 r = pi - v3(q[u]);
                 float qq = sqr(H2 - dot(r, r));
-                acc = acc + (r * qq) / q[u].w;
+                acc = acc + Math<F>::div(r * qq, q[u].w);
             }
         }
     }
@@ -823,19 +824,21 @@ __device__ __forceinline__ void normals_one(int i, int n, const float4* __restri
     normal[i] = f4(o, 0.0f);
 }
 
+template <bool F>
 __global__ __launch_bounds__(256) void k_normals(int n, const float4* __restrict__ prho,
                                                  const int* __restrict__ nl, const int* __restrict__ nc, float H2,
                                                  float w_grad, float M, float H, float4* __restrict__ normal) {
     int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    normals_one(i, n, prho, nl, nc, H2, w_grad, M, H, normal);
+    normals_one<F>(i, n, prho, nl, nc, H2, w_grad, M, H, normal);
 }
 
 void launch_normals(int n, const float4* prho, const int* nl, const int* nc, float H2, float w_grad, float M,
-                    float H, float4* normal, hipStream_t s, Profiler* prof) {
+                    float H, float4* normal, hipStream_t s, Profiler* prof, bool fast) {
     if (n <= 0) return;
     PROF_BEGIN(prof, "normals", s);
-    hipLaunchKernelGGL(k_normals, dim3((n + 255) / 256), dim3(256), 0, s, n, prho, nl, nc, H2, w_grad, M, H, normal);
+    hipLaunchKernelGGL(fast ? k_normals<true> : k_normals<false>, dim3((n + 255) / 256), dim3(256), 0, s, n, prho, nl,
+                       nc, H2, w_grad, M, H, normal);
     PROF_END(prof, s);
 }
 

@@ -74,6 +74,7 @@ struct PciParams {
     float rho0_host, eta;                // exit test operands (host floats)
     int min_iter, max_iter;
     unsigned long long barrier_ticks;    // bound of each grid-barrier wait, device wall-clock ticks
+    int fast;                            // 1: Math<true> builds of the kernels (SPH_FAST_MATH)
 };
 
 struct WcParams {
@@ -116,8 +117,9 @@ void launch_lists_density(bool wcsph, int n, const float4* pos_s, const int2* fR
                           const float* bphi_s, const int2* bR, const uint32_t* bnear, const GridDev& g,
                           const DensityParams& P, int* nl, int* nc, int* bnl, int* bnc, float* density,
                           float* pressure, float4* prho, DevState* st, hipStream_t s, Profiler* prof);
+// fast: the Math<true> build (SPH_FAST_MATH)
 void launch_normals(int n, const float4* prho, const int* nl, const int* nc, float H2, float w_grad, float M,
-                    float H, float4* normal, hipStream_t s, Profiler* prof);
+                    float H, float4* normal, hipStream_t s, Profiler* prof, bool fast = false);
 void launch_step_begin(DevState* st, hipStream_t s);
 void launch_fill_u32(void* p, size_t n, uint32_t v, hipStream_t s);
 void launch_stats(int n, const int* nc, const int* bnc, const float4* pos, unsigned long long* acc, hipStream_t s);

@@ -23,32 +23,37 @@
 namespace sph {
 
 // acc = g + (pressure_force + other_force) / PARTICLE_MASS (pcisph.cl:260,321)
+template <bool F>
 __device__ __forceinline__ V3 pci_accel(V3 fp, V3 fo, const PciParams& P) {
-    return {P.gx + (fp.x + fo.x) / P.M, P.gy + (fp.y + fo.y) / P.M, P.gz + (fp.z + fo.z) / P.M};
+    using M = Math<F>;
+    return {P.gx + M::div(fp.x + fo.x, P.M), P.gy + M::div(fp.y + fo.y, P.M), P.gz + M::div(fp.z + fo.z, P.M)};
 }
 
 // predict_pos body (pcisph.cl:319-343)
+template <bool F>
 __device__ __forceinline__ V3 pci_predict(V3 x, V3 v, V3 acc, const PciParams& P) {
     v = v + clamp3(acc * P.dt, -P.max_vel, P.max_vel);
     x = x + v * P.dt;
-    float vn = length(v);
+    float vn = Math<F>::length(v);
     V3 cp = {cl_clamp(x.x, P.lo_x, P.hi_x), cl_clamp(x.y, P.lo_y, 10.0f), cl_clamp(x.z, P.lo_z, P.hi_z)};
-    float d = length(cp - x);
+    float d = Math<F>::length(cp - x);
     if (d > 0.0f && vn > 0.0f) x = cp;
     return x;
 }
 
 // predict_vel_n_pos body (pcisph.cl:258-283)
+template <bool F>
 __device__ __forceinline__ void pci_integrate(V3& x, V3& v, V3 acc, const PciParams& P) {
+    using M = Math<F>;
     v = v + clamp3(acc * P.dt, -P.max_vel, P.max_vel);
     x = x + v * P.dt;
-    float vn = length(v);
+    float vn = M::length(v);
     V3 cp = {cl_clamp(x.x, P.lo_x, P.hi_x), cl_clamp(x.y, P.lo_y, 10.0f), cl_clamp(x.z, P.lo_z, P.hi_z)};
     V3 dv = cp - x;
-    float d = length(dv);
+    float d = M::length(dv);
     if (d > 0.0f && vn > 0.0f) {
-        V3 nrm = normalize(V3{cl_sign(dv.x), cl_sign(dv.y), cl_sign(dv.z)});
-        float rest = d / (P.dt * vn);
+        V3 nrm = M::normalize(V3{cl_sign(dv.x), cl_sign(dv.y), cl_sign(dv.z)});
+        float rest = M::div(d, P.dt * vn);
         float k = (1.0f + rest) * dot(v, nrm);
         v = v - k * nrm;
         x = cp;
@@ -70,6 +75,7 @@ __device__ __forceinline__ float st_kernel(float r, float h, float c) {
 
 constexpr int FORCE_BATCH = 4;
 
+template <bool F>
 __global__ __launch_bounds__(256) void k_forces_predict(int n, const float4* __restrict__ prho,
                                                         const float4* __restrict__ vel_s,
                                                         const float4* __restrict__ normal,
@@ -104,16 +110,16 @@ __global__ __launch_bounds__(256) void k_forces_predict(int n, const float4* __r
         for (int u = 0; u < FORCE_BATCH; ++u) {
             if (k0 + u < len) {
                 V3 r = pi - v3(qp[u]);
-                float l = length(r);
+                float l = Math<F>::length(r);
                 float dj = qp[u].w;
-                fv = fv + (v3(qv[u]) - vi) * ((P.H - l) / dj);
-                float corr = (2.0f * P.rho0_d) / (di + dj);
+                fv = fv + (v3(qv[u]) - vi) * Math<F>::div(P.H - l, dj);
+                float corr = Math<F>::div(2.0f * P.rho0_d, di + dj);
                 fcurv = fcurv + (ni - v3(qn[u])) * corr;
-                fcoh = fcoh + (normalize(r) * st_kernel(l, P.H, P.st_term)) * corr;
+                fcoh = fcoh + (Math<F>::normalize(r) * st_kernel(l, P.H, P.st_term)) * corr;
             }
         }
     }
-    fv = fv * ((P.k_visc * sqr(P.M)) * (P.visc_lapl / di));
+    fv = fv * ((P.k_visc * sqr(P.M)) * Math<F>::div(P.visc_lapl, di));
     fcoh = fcoh * (P.M * P.st_main);
     V3 ft = ((-P.sigma) * P.M) * (fcurv + fcoh);
     V3 fo = fv + ft;
@@ -121,13 +127,14 @@ __global__ __launch_bounds__(256) void k_forces_predict(int n, const float4* __r
     // first predict_pos: the pressure force buffer is zero-filled at this
     // point (pcisphsimluation.cpp:126)
     V3 zero = {0.0f, 0.0f, 0.0f};
-    pos_pred[i] = f4(pci_predict(pi, vi, pci_accel(zero, fo, P), P), 1.0f);
+    pos_pred[i] = f4(pci_predict<F>(pi, vi, pci_accel<F>(zero, fo, P), P), 1.0f);
 }
 
 // update_pressure also packs (x, y, z, p/rho^2) of every particle for the
 // pressure force that follows: p_j/SQR(rho_j) of pcisph.cl:482 and
 // C = p_i/SQR(rho_i) of pcisph.cl:452 are this same value.
 // One correction's update_pressure over the whole grid (grid-stride loop).
+template <bool F>
 __device__ __forceinline__ void update_pressure_phase(int n, int iter, const float4* __restrict__ pos_pred,
                                                       const float4* __restrict__ prho,
                                                       const int* __restrict__ nl, const int* __restrict__ nc,
@@ -190,7 +197,7 @@ __device__ __forceinline__ void update_pressure_phase(int n, int iter, const flo
         float p = p0 + dv * P.delta;
         pressure[i] = p;
         float4 pr = prho[i];
-        pc[i] = make_float4(pr.x, pr.y, pr.z, p / sqr(pr.w));
+        pc[i] = make_float4(pr.x, pr.y, pr.z, Math<F>::div(p, sqr(pr.w)));
     }
     // dv >= +0, so its bits order like the values
     uint32_t bits = __float_as_uint(dvmax);
@@ -207,6 +214,7 @@ __device__ __forceinline__ void update_pressure_phase(int n, int iter, const flo
     }
 }
 
+template <bool F>
 __global__ __launch_bounds__(256) void k_update_pressure(int n, int iter, const float4* __restrict__ pos_pred,
                                                          const float4* __restrict__ prho,
                                                          const int* __restrict__ nl, const int* __restrict__ nc,
@@ -217,12 +225,14 @@ __global__ __launch_bounds__(256) void k_update_pressure(int n, int iter, const 
                                                          float4* __restrict__ pc, DevState* st) {
     int done = st->done_iter;
     if (done != 0 && done < iter) return;
-    update_pressure_phase(n, iter, pos_pred, prho, nl, nc, bnl, bnc, bpos_s, bphi_s, has_b, P, err, pressure, pc, st);
+    update_pressure_phase<F>(n, iter, pos_pred, prho, nl, nc, bnl, bnc, bpos_s, bphi_s, has_b, P, err, pressure, pc,
+                             st);
 }
 
 // One correction's pressure force and prediction (or final integration)
 // over the whole grid. Reads the maximum error that every block of the
 // update_pressure phase has merged; block 0 records the end of the loop.
+template <bool F>
 __device__ __forceinline__ void pressure_force_phase(
     int n, int iter, const float4* __restrict__ pc, const float4* __restrict__ vel_s,
     const float4* __restrict__ f_other, const int* __restrict__ nl, const int* __restrict__ nc,
@@ -259,9 +269,9 @@ __device__ __forceinline__ void pressure_force_phase(
         for (int u = 0; u < GATHER_BATCH; ++u) {
             if (k0 + u < len) {
                 V3 r = xi - v3(q[u]);
-                float l = length(r);
+                float l = Math<F>::length(r);
                 if (l >= 1e-5f) {
-                    V3 gw = (r * (1.0f / l)) * sqr(P.H - l);
+                    V3 gw = (r * Math<F>::div(1.0f, l)) * sqr(P.H - l);
                     f = f + (C + q[u].w) * gw;
                 }
             }
@@ -287,8 +297,8 @@ __device__ __forceinline__ void pressure_force_phase(
             for (int u = 0; u < BOUND_BATCH; ++u) {
                 if (k0 + u < blen) {
                     V3 r = xi - v3(q[u]);
-                    float l = length(r);
-                    V3 gw = (r * (1.0f / l)) * sqr(P.H - l);
+                    float l = Math<F>::length(r);
+                    V3 gw = (r * Math<F>::div(1.0f, l)) * sqr(P.H - l);
                     fb = fb + (w[u] * (2.0f * C)) * gw;
                 }
             }
@@ -298,18 +308,19 @@ __device__ __forceinline__ void pressure_force_phase(
     V3 fp = f + fb;
     f_press[i] = f4(fp, 0.0f);
     V3 vi = v3(vel_s[i]);
-    V3 acc = pci_accel(fp, v3(f_other[i]), P);
+    V3 acc = pci_accel<F>(fp, v3(f_other[i]), P);
     if (!last) {
-        pos_pred[i] = f4(pci_predict(xi, vi, acc, P), 1.0f);
+        pos_pred[i] = f4(pci_predict<F>(xi, vi, acc, P), 1.0f);
     } else {
         V3 x = xi, v = vi;
-        pci_integrate(x, v, acc, P);
+        pci_integrate<F>(x, v, acc, P);
         pos_out[i] = f4(x, 1.0f);
         vel_out[i] = f4(v, 0.0f);
     }
 }
 }
 
+template <bool F>
 __global__ __launch_bounds__(256) void k_pressure_force(
     int n, int iter, const float4* __restrict__ pc, const float4* __restrict__ vel_s,
     const float4* __restrict__ f_other, const int* __restrict__ nl, const int* __restrict__ nc,
@@ -318,8 +329,8 @@ __global__ __launch_bounds__(256) void k_pressure_force(
     float4* __restrict__ pos_pred, float4* __restrict__ pos_out, float4* __restrict__ vel_out, DevState* st) {
     int done = st->done_iter;
     if (done != 0 && done < iter) return;
-    pressure_force_phase(n, iter, pc, vel_s, f_other, nl, nc, bnl, bnc, bpos_s, bphi_s, has_b, P, f_press, pos_pred,
-                         pos_out, vel_out, st);
+    pressure_force_phase<F>(n, iter, pc, vel_s, f_other, nl, nc, bnl, bnc, bpos_s, bphi_s, has_b, P, f_press, pos_pred,
+                            pos_out, vel_out, st);
 }
 
 // Corrections min_iter + 1 .. max_iter in one launch. They run only when the
@@ -331,6 +342,7 @@ __global__ __launch_bounds__(256) void k_pressure_force(
 // once. The exit decision is read after a barrier, so all blocks leave the
 // loop at the same correction; a barrier that times out ends the launch and
 // is reported by the step statistics.
+template <bool F>
 __global__ __launch_bounds__(256) void k_correction_tail(
     int n, int first, const float4* __restrict__ prho, const float4* __restrict__ vel_s,
     const float4* __restrict__ f_other, const int* __restrict__ nl, const int* __restrict__ nc,
@@ -340,16 +352,17 @@ __global__ __launch_bounds__(256) void k_correction_tail(
     for (int iter = first; iter <= P.max_iter; ++iter) {
         const int done = __hip_atomic_load(&st->done_iter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (done != 0 && done < iter) return;
-        update_pressure_phase(n, iter, pos_pred, prho, nl, nc, bnl, bnc, bpos_s, bphi_s, has_b, P, err, pressure, pc,
-                              st);
+        update_pressure_phase<F>(n, iter, pos_pred, prho, nl, nc, bnl, bnc, bpos_s, bphi_s, has_b, P, err, pressure,
+                                 pc, st);
         if (!grid_barrier(st, P.barrier_ticks)) return;
-        pressure_force_phase(n, iter, pc, vel_s, f_other, nl, nc, bnl, bnc, bpos_s, bphi_s, has_b, P, f_press,
-                             pos_pred, pos_out, vel_out, st);
+        pressure_force_phase<F>(n, iter, pc, vel_s, f_other, nl, nc, bnl, bnc, bpos_s, bphi_s, has_b, P, f_press,
+                                pos_pred, pos_out, vel_out, st);
         if (!grid_barrier(st, P.barrier_ticks)) return;
     }
 }
 
 // Final integration with a zero pressure force, for a loop of zero corrections.
+template <bool F>
 __global__ __launch_bounds__(256) void k_integrate_only(int n, const float4* __restrict__ pos_s,
                                                         const float4* __restrict__ vel_s,
                                                         const float4* __restrict__ f_other, PciParams P,
@@ -358,7 +371,7 @@ __global__ __launch_bounds__(256) void k_integrate_only(int n, const float4* __r
     if (i >= n) return;
     V3 x = v3(pos_s[i]), v = v3(vel_s[i]);
     V3 zero = {0.0f, 0.0f, 0.0f};
-    pci_integrate(x, v, pci_accel(zero, v3(f_other[i]), P), P);
+    pci_integrate<F>(x, v, pci_accel<F>(zero, v3(f_other[i]), P), P);
     pos_out[i] = f4(x, 1.0f);
     vel_out[i] = f4(v, 0.0f);
 }
@@ -370,19 +383,22 @@ static int correction_blocks(int n) { return (n + 255) / 256; }
 // Upper bound of the tail grid (two workgroups per CU of a 256-CU device).
 constexpr int TAIL_MAX_BLOCKS = 512;
 
-// Blocks of k_correction_tail that the current device holds resident at once
-// (cached per device), so that its grid barriers can complete.
-static int resident_tail_blocks() {
-    static int cached[64] = {0};
+// Blocks of k_correction_tail (the IEEE or the fast build) that the current
+// device holds resident at once (cached per device), so that its grid
+// barriers can complete.
+static int resident_tail_blocks(bool fast) {
+    static int cached[64][2] = {};
     int dev = 0;
     hip_check(hipGetDevice(&dev), "hipGetDevice");
-    if (dev >= 0 && dev < 64 && cached[dev] > 0) return cached[dev];
+    if (dev >= 0 && dev < 64 && cached[dev][fast] > 0) return cached[dev][fast];
     int per_cu = 0, cus = 0;
-    hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_correction_tail, 256, 0), "occupancy");
+    hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                  &per_cu, fast ? k_correction_tail<true> : k_correction_tail<false>, 256, 0),
+              "occupancy");
     hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "device attribute");
     const int r = per_cu * cus;
     if (r < 1) throw Error(SPH_ERR_HIP, "k_correction_tail cannot be resident on this device");
-    if (dev >= 0 && dev < 64) cached[dev] = r;
+    if (dev >= 0 && dev < 64) cached[dev][fast] = r;
     return r;
 }
 
@@ -391,8 +407,8 @@ void launch_forces_predict(int n, const float4* prho, const float4* vel_s, const
                            Profiler* prof) {
     if (n <= 0) return;
     PROF_BEGIN(prof, "forces_predict", s);
-    hipLaunchKernelGGL(k_forces_predict, dim3((n + 255) / 256), dim3(256), 0, s, n, prho, vel_s, normal, nl, nc, P,
-                       f_other, pos_pred);
+    hipLaunchKernelGGL(P.fast ? k_forces_predict<true> : k_forces_predict<false>, dim3((n + 255) / 256), dim3(256), 0,
+                       s, n, prho, vel_s, normal, nl, nc, P, f_other, pos_pred);
     PROF_END(prof, s);
 }
 
@@ -402,8 +418,9 @@ void launch_update_pressure(int n, int iter, const float4* pos_pred, const float
                             hipStream_t s, Profiler* prof) {
     if (n <= 0) return;
     PROF_BEGIN(prof, "update_pressure", s);
-    hipLaunchKernelGGL(k_update_pressure, dim3(correction_blocks(n)), dim3(256), 0, s, n, iter, pos_pred, prho, nl, nc,
-                       bnl, bnc, bpos_s, bphi_s, has_b, P, err, pressure, pc, st);
+    hipLaunchKernelGGL(P.fast ? k_update_pressure<true> : k_update_pressure<false>, dim3(correction_blocks(n)),
+                       dim3(256), 0, s, n, iter, pos_pred, prho, nl, nc, bnl, bnc, bpos_s, bphi_s, has_b, P, err,
+                       pressure, pc, st);
     PROF_END(prof, s);
 }
 
@@ -413,8 +430,9 @@ void launch_pressure_force(int n, int iter, const float4* pc, const float4* vel_
                            float4* pos_out, float4* vel_out, DevState* st, hipStream_t s, Profiler* prof) {
     if (n <= 0) return;
     PROF_BEGIN(prof, "pressure_force", s);
-    hipLaunchKernelGGL(k_pressure_force, dim3(correction_blocks(n)), dim3(256), 0, s, n, iter, pc, vel_s, f_other, nl, nc,
-                       bnl, bnc, bpos_s, bphi_s, has_b, P, f_press, pos_pred, pos_out, vel_out, st);
+    hipLaunchKernelGGL(P.fast ? k_pressure_force<true> : k_pressure_force<false>, dim3(correction_blocks(n)),
+                       dim3(256), 0, s, n, iter, pc, vel_s, f_other, nl, nc, bnl, bnc, bpos_s, bphi_s, has_b, P,
+                       f_press, pos_pred, pos_out, vel_out, st);
     PROF_END(prof, s);
 }
 
@@ -425,10 +443,11 @@ void launch_correction_tail(int n, int first, const float4* prho, const float4* 
                             DevState* st, hipStream_t s, Profiler* prof) {
     if (n <= 0 || first > P.max_iter) return;
     int blocks = correction_blocks(n);
-    const int cap = TAIL_MAX_BLOCKS < resident_tail_blocks() ? TAIL_MAX_BLOCKS : resident_tail_blocks();
+    const int resident = resident_tail_blocks(P.fast != 0);
+    const int cap = TAIL_MAX_BLOCKS < resident ? TAIL_MAX_BLOCKS : resident;
     if (blocks > cap) blocks = cap;
     PROF_BEGIN(prof, "correction_tail", s);
-    hipLaunchKernelGGL(k_correction_tail, dim3(blocks), dim3(256), 0, s, n, first, prho, vel_s, f_other, nl, nc, bnl,
+    hipLaunchKernelGGL(P.fast ? k_correction_tail<true> : k_correction_tail<false>, dim3(blocks), dim3(256), 0, s, n, first, prho, vel_s, f_other, nl, nc, bnl,
                        bnc, bpos_s, bphi_s, has_b, P, err, pressure, pc, f_press, pos_pred, pos_out, vel_out, st);
     PROF_END(prof, s);
 }
@@ -438,8 +457,8 @@ void launch_pci_integrate_only(int n, const float4* pos_s, const float4* vel_s, 
                                Profiler* prof) {
     if (n <= 0) return;
     PROF_BEGIN(prof, "integrate", s);
-    hipLaunchKernelGGL(k_integrate_only, dim3((n + 255) / 256), dim3(256), 0, s, n, pos_s, vel_s, f_other, P, pos_out,
-                       vel_out);
+    hipLaunchKernelGGL(P.fast ? k_integrate_only<true> : k_integrate_only<false>, dim3((n + 255) / 256), dim3(256), 0,
+                       s, n, pos_s, vel_s, f_other, P, pos_out, vel_out);
     PROF_END(prof, s);
 }
 

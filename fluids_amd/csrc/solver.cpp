@@ -840,6 +840,7 @@ void PCISPHSimulation::fill_params() {
     P.eta = s_.pcisph_error_ratio;
     P.min_iter = 3;
     P.max_iter = s_.pcisph_max_iterations;
+    P.fast = (s_.flags & SPH_FAST_MATH) ? 1 : 0;
     P.barrier_ticks = barrier_timeout_s_ > 0.0
                           ? (unsigned long long)(barrier_timeout_s_ * device_wall_clock_hz())
                           : 0ull;
@@ -877,7 +878,7 @@ void PCISPHSimulation::simulate_step(int min_iter, int max_iter) {
                          d_nl.get(), d_nc.get(), d_bnl.get(), d_bnc.get(), d_density.get(), nullptr, d_prho.get(),
                          d_state.get(), s, &prof_);
     launch_normals(n_, d_prho.get(), d_nl.get(), d_nc.get(), P_.H2, P_.w_grad, P_.M, P_.H, d_normal.get(), s,
-                   &prof_);
+                   &prof_, P_.fast != 0);
     if (max_iter <= 0) {
         launch_fill_u32(d_pressure.get(), n_, 0u, s);
         launch_fill_u32(d_f_press.get(), (size_t)n_ * 4, 0u, s);

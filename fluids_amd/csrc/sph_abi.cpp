@@ -105,6 +105,9 @@ void validate(const sph_settings* s) {
     if (!(s->fluid_support_radius > 0.0f) || !(s->fluid_particle_radius > 0.0f))
         throw Error(SPH_ERR_INVALID, "radii must be positive");
     if (!(s->time_step > 0.0f)) throw Error(SPH_ERR_INVALID, "time_step must be positive");
+    if (s->flags & ~(SPH_FIX_WCSPH_DENSITY | SPH_FAST_MATH)) throw Error(SPH_ERR_INVALID, "unknown flag bits");
+    if ((s->flags & SPH_FAST_MATH) && s->method != SPH_PCISPH)
+        throw Error(SPH_ERR_INVALID, "SPH_FAST_MATH applies to PCISPH only");
 }
 
 SimulationBase& sim_of(sph_handle* h) {

@@ -117,6 +117,42 @@ __device__ __forceinline__ V3 clamp3(V3 a, float lo, float hi) {
 }
 __device__ __forceinline__ float4 f4(V3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
 
+// Division and square root of one kernel build. Math<false> is the IEEE
+// arithmetic above (correctly rounded, bit-reproducible). Math<true> follows
+// the reference's -cl-fast-relaxed-math builds (pcisphsimluation.cpp:496,
+// normals through grid.cpp:82): a / b = a * rcp(b), sqrt and 1/sqrt by the
+// hardware's v_rcp_f32, v_sqrt_f32 and v_rsq_f32 (each within about 1 ulp);
+// additions and products stay uncontracted.
+template <bool FAST> struct Math {
+    static __device__ __forceinline__ float div(float a, float b) {
+        if constexpr (FAST) return a * __builtin_amdgcn_rcpf(b);
+        else return a / b;
+    }
+    static __device__ __forceinline__ float sqrt(float x) {
+        if constexpr (FAST) return __builtin_amdgcn_sqrtf(x);
+        else return sqrtf(x);
+    }
+    static __device__ __forceinline__ V3 div(V3 a, float s) {
+        if constexpr (FAST) {
+            const float inv = __builtin_amdgcn_rcpf(s);
+            return {a.x * inv, a.y * inv, a.z * inv};
+        } else {
+            return a / s;
+        }
+    }
+    static __device__ __forceinline__ float length(V3 a) { return sqrt(dot(a, a)); }
+    static __device__ __forceinline__ V3 normalize(V3 a) {
+        if constexpr (FAST) {
+            float d = dot(a, a);
+            if (d == 0.0f) return a;
+            float inv = __builtin_amdgcn_rsqf(d);
+            return {a.x * inv, a.y * inv, a.z * inv};
+        } else {
+            return sph::normalize(a);
+        }
+    }
+};
+
 // get_grid_coordinates (kernels/grid.h:74-91)
 __device__ __forceinline__ void grid_coords(float4 p, const GridDev& g, int& cx, int& cy, int& cz) {
     cx = (int)floorf((p.x + g.half) / g.cell);

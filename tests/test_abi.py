@@ -74,6 +74,27 @@ def test_last_error_is_per_call():
     assert np.float32(out.rest_density) == np.float32(998.29)  # physicssettings.h:18
 
 
+def test_flag_validation():
+    """Unknown flag bits are rejected, and SPH_FAST_MATH is a PCISPH option:
+    a WCSPH scene, a batch scene and the adjoint (both WCSPH) reject it
+    before any device call."""
+    L = _lib.lib()
+    s = _lib.SphSettings()
+    assert L.sph_default_settings(C.byref(s)) == 0
+    s.method = _lib.WCSPH
+    s.flags = _lib.FAST_MATH
+    assert not L.sph_create(C.byref(s))
+    assert b"PCISPH only" in L.sph_last_error()
+    pos = np.zeros((1, 4), np.float32)
+    p = pos.ctypes.data_as(C.POINTER(C.c_float))
+    assert not L.sph_adjoint_create(C.byref(s), 1.0, 1.0, 1.0, p, p, p, 1, 1)
+    assert b"PCISPH only" in L.sph_last_error()
+    s.method = _lib.PCISPH
+    s.flags = 8
+    assert not L.sph_create(C.byref(s))
+    assert b"unknown flag bits" in L.sph_last_error()
+
+
 def test_samplers_match_oracle():
     assert np.array_equal(fishtank_sampling(2, 1, 1, 0.02), oracle.fishtank(2, 1, 1, 0.02))
     assert np.array_equal(fishtank_sampling(2, 1, 1, 0.005), oracle.fishtank(2, 1, 1, 0.005))
