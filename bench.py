@@ -47,7 +47,7 @@ def kernel_bytes(name, K, Kb, n_it):
         "hash": 24.0,
         "radix_scatter": 16.0 / 4,  # the reference's sort counts 16 B/p over all passes
         "radix_hist": 0.0,
-        "radix_scan": 0.0,
+        "radix_rowscan": 0.0,
         "gather_ranges": 72.0 + 21.0,
         "lists_density": 68.0 + 8 * K + 8 * Kb,
         "normals": 40.0 + 4 * K,
