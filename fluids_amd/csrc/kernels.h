@@ -83,12 +83,6 @@ struct WcParams {
     float lo_x, hi_x, lo_y, lo_z, hi_z;
 };
 
-struct BodyDev {
-    int origin, count, partial_offset, nblocks;
-    float com[3];
-    float phi_mass;  // unused slot kept for alignment
-};
-
 // grid.hip
 int sort_passes(int key_bits);
 // passes of a sort whose last pass orders by scene id (sid_bits > 0)
