@@ -650,7 +650,7 @@ __global__ __launch_bounds__(LISTS_THREADS, 8) void k_lists_density(
 constexpr int SPLIT = 4;
 constexpr int SPLIT_CELLS = 7;  // cells per part; the last part has 6
 constexpr int SPLIT_THREADS = 64;
-constexpr int SPLIT_MAX_N = 1 << 17;  // the split walk serves scenes up to this many particles
+constexpr int SPLIT_MAX_N = 1 << 15;  // the split walk serves scenes up to this many particles
 
 template <bool WCSPH>
 __global__ __launch_bounds__(SPLIT_THREADS) void k_lists_density_split(

@@ -161,7 +161,8 @@ def c2_pcisph_double_dambreak(seed: int = 202, spacing: float = 0.01, with_cube:
 
 
 def _c3_dims(n: int):
-    table = {1_000_000: (250, 50, 80), 4_000_000: (250, 100, 160)}
+    # 125k and 500k: one GPU's share of the 1M and 4M scenes over 8 GPUs
+    table = {125_000: (250, 25, 20), 500_000: (250, 40, 50), 1_000_000: (250, 50, 80), 4_000_000: (250, 100, 160)}
     if n in table:
         return table[n]
     # generic: a slab 250 wide and 5:8 in y:z

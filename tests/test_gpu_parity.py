@@ -128,6 +128,27 @@ def test_wcsph_single_step_parity(fix):
     _twin_errors(orc, ["density", "pressure", "normal", "accel", "pos"], "wcsph")
 
 
+@pytest.mark.parametrize("method", [1, 0])
+def test_mid_size_parity(method):
+    """A 50k-particle scene: from 2^15 particles up the lists + density pass
+    runs one thread per particle (smaller scenes split each walk over four
+    lanes). Two steps against the oracle from the same state, every field
+    bit for bit."""
+    make = scenes.small_pcisph if method == 1 else scenes.small_wcsph
+    sc = make(n_side=(40, 25, 50), spacing=0.005)
+    print("case mid-size", "pcisph" if method == 1 else "wcsph", sc.n_fluid, sc.n_boundary, flush=True)
+    sim = scenes.build(sc)
+    orc = build_oracle(sc)
+    fields = PCI_FIELDS if method == 1 else WC_FIELDS
+    for step in range(2):
+        _same_state(sim, orc, method)
+        sim.step(1)
+        orc.step(1)
+        rep = _compare(sim, orc, fields, f"mid-size step {step}")
+        bad = {f: rep[f + "_bits_differing"] for f in fields if rep[f + "_bits_differing"]}
+        assert not bad, bad
+
+
 def test_multistep_trajectory_bitexact():
     """Ten steps from the same initial state stay bit-identical (no reset of
     the state in between): the sort order chain of the reference (the input
