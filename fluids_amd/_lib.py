@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(HERE, "libsph.so")
 SPH_OK = 0
 WCSPH, PCISPH = 0, 1
 FIX_WCSPH_DENSITY = 1
-FAST_MATH = 2  # PCISPH kernels with approximate division and square root (SPH_FAST_MATH)
+FAST_MATH = 2  # single-scene kernels with approximate division and square root (SPH_FAST_MATH)
 LIST_CAP = 50
 
 FIELDS = {

@@ -81,6 +81,7 @@ struct WcParams {
     float h, M, spiky_grad, visc_lapl, k_visc, sigma, rho0, st_main, st_term;
     float dt, max_vel, gx, gy, gz;
     float lo_x, hi_x, lo_y, lo_z, hi_z;
+    int fast;  // 1: Math<true> build of the single-scene kernels (SPH_FAST_MATH); batches use the IEEE build
 };
 
 // grid.hip

@@ -965,6 +965,7 @@ WcStepParams wcsph_step_params(const Settings& s, const float container[3]) {
     P.lo_y = -container[1] / 2.0f;
     P.lo_z = -container[2] / 2.0f;
     P.hi_z = container[2] / 2.0f;
+    P.fast = (s.flags & SPH_FAST_MATH) ? 1 : 0;
     return o;
 }
 
@@ -986,7 +987,7 @@ void WCSPHSimulation::simulate() {
                          d_nl.get(), d_nc.get(), d_bnl.get(), d_bnc.get(), d_density.get(), d_pressure.get(),
                          d_prho.get(), d_state.get(), s, &prof_);
     launch_normals(n_, d_prho.get(), d_nl.get(), d_nc.get(), W.nH2, W.nw_grad, W.nM, W.nH, d_normal.get(), s,
-                   &prof_);
+                   &prof_, W.P.fast != 0);
     boundary_.apply_fluid_forces();
     launch_accel_integrate(n_, d_prho.get(), d_vel_s.get(), d_vh_s.get(), d_pressure.get(), d_normal.get(),
                            d_nl.get(), d_nc.get(), d_bnl.get(), d_bnc.get(), boundary_.d_pos_s.get(),

@@ -106,8 +106,13 @@ void validate(const sph_settings* s) {
         throw Error(SPH_ERR_INVALID, "radii must be positive");
     if (!(s->time_step > 0.0f)) throw Error(SPH_ERR_INVALID, "time_step must be positive");
     if (s->flags & ~(SPH_FIX_WCSPH_DENSITY | SPH_FAST_MATH)) throw Error(SPH_ERR_INVALID, "unknown flag bits");
-    if ((s->flags & SPH_FAST_MATH) && s->method != SPH_PCISPH)
-        throw Error(SPH_ERR_INVALID, "SPH_FAST_MATH applies to PCISPH only");
+}
+
+// Batches and the adjoint run the IEEE kernels only (the adjoint's
+// derivatives are those of the IEEE step).
+void validate_ieee_only(const sph_settings* s, const char* who) {
+    validate(s);
+    if (s->flags & SPH_FAST_MATH) throw Error(SPH_ERR_INVALID, std::string(who) + ": SPH_FAST_MATH is not supported");
 }
 
 SimulationBase& sim_of(sph_handle* h) {
@@ -569,7 +574,7 @@ int sph_batch_destroy(sph_batch* b) {
 int sph_batch_add_scene(sph_batch* b, const sph_settings* s, float width, float height, float depth,
                         const float* xyzw, int n) {
     return guard([&] {
-        validate(s);
+        validate_ieee_only(s, "batch");
         const float c[3] = {width, height, depth};
         return batch_of(b).add_scene(to_settings(s), c, xyzw, n);
     });
@@ -721,7 +726,7 @@ sph_adjoint* sph_adjoint_create(const sph_settings* s, float width, float height
                                 const float* vel, const float* vel_half, int n, int checkpoint_every) {
     sph_adjoint* out = nullptr;
     int rc = guard([&] {
-        validate(s);
+        validate_ieee_only(s, "adjoint");
         const float c[3] = {width, height, depth};
         std::unique_ptr<WcAdjoint> a(new WcAdjoint(to_settings(s), c, pos, vel, vel_half, n, checkpoint_every));
         out = new sph_adjoint{std::move(a)};

@@ -26,7 +26,8 @@ __device__ __forceinline__ float st_kernel_wc(float r, float h, float c) {
 constexpr int WC_BATCH = 4;
 
 // One particle of compute_acceleration + predict_vel_n_pos; n is the list
-// stride (the total particle count).
+// stride (the total particle count). F: the Math<true> build (SPH_FAST_MATH).
+template <bool F = false>
 __device__ __forceinline__ void accel_integrate_one(
     int i, int n, const float4* __restrict__ prho, const float4* __restrict__ vel_s, const float4* __restrict__ vh_s,
     const float* __restrict__ pressure, const float4* __restrict__ normal, const int* __restrict__ nl,
@@ -38,7 +39,8 @@ __device__ __forceinline__ void accel_integrate_one(
     V3 pi = v3(p4), vi = v3(vel_s[i]);
     float di = p4.w, pri = pressure[i];
     V3 ni = v3(normal[i]);
-    float C = pri / sqr(di);
+    using M = Math<F>;
+    float C = M::div(pri, sqr(di));
     const float h = P.h;
     V3 bacc = {0.0f, 0.0f, 0.0f};
     // boundary terms in list order; indices past blen read boundary particle
@@ -59,9 +61,9 @@ __device__ __forceinline__ void accel_integrate_one(
         for (int u = 0; u < BOUND_BATCH; ++u) {
             if (k0 + u < blen) {
                 V3 r = pi - v3(q[u]);
-                float rn = length(r);
+                float rn = M::length(r);
                 V3 t = (-w[u]) * r;
-                t = t * (P.spiky_grad / rn);
+                t = t * M::div(P.spiky_grad, rn);
                 t = t * sqr(h - rn);
                 t = t * 2.0f;
                 t = t * C;
@@ -95,14 +97,14 @@ __device__ __forceinline__ void accel_integrate_one(
             if (k0 + u < len) {
                 float dj = qp[u].w, prj = qpr[u];
                 V3 r = pi - v3(qp[u]);
-                float rn = length(r);
+                float rn = M::length(r);
                 // wcsph.cl:152 compares against the double literal 0.00001
                 if (di > 0.0f && dj > 0.0f && (double)rn > 0.00001) {
-                    pacc = pacc + ((r * (P.spiky_grad / rn)) * sqr(h - rn)) * (C + prj / sqr(dj));
-                    vacc = vacc + ((v3(qv[u]) - vi) * (h - rn)) / dj;
-                    float corr = (2.0f * P.rho0) / (di + dj);
+                    pacc = pacc + ((r * M::div(P.spiky_grad, rn)) * sqr(h - rn)) * (C + M::div(prj, sqr(dj)));
+                    vacc = vacc + M::div((v3(qv[u]) - vi) * (h - rn), dj);
+                    float corr = M::div(2.0f * P.rho0, di + dj);
                     fcurv = fcurv + (ni - v3(qn[u])) * corr;
-                    fcoh = fcoh + (normalize(r) * st_kernel_wc(rn, h, P.st_term)) * corr;
+                    fcoh = fcoh + (M::normalize(r) * st_kernel_wc(rn, h, P.st_term)) * corr;
                 }
             }
         }
@@ -121,13 +123,13 @@ __device__ __forceinline__ void accel_integrate_one(
     V3 vh = v3(vh_s[i]);
     vh = vh + P.dt * a;
     x = x + P.dt * clamp3(vh + P.dt * a, -P.max_vel, P.max_vel);
-    float vn = length(vh);
+    float vn = M::length(vh);
     V3 cp = {cl_clamp(x.x, P.lo_x, P.hi_x), cl_clamp(x.y, P.lo_y, 10.0f), cl_clamp(x.z, P.lo_z, P.hi_z)};
     V3 dvec = cp - x;
-    float d = length(dvec);
+    float d = M::length(dvec);
     if (d > 0.0f && vn > 0.0f) {
-        V3 nrm = normalize(V3{cl_sign(dvec.x), cl_sign(dvec.y), cl_sign(dvec.z)});
-        float rest = (1.0f * d) / (P.dt * vn);
+        V3 nrm = M::normalize(V3{cl_sign(dvec.x), cl_sign(dvec.y), cl_sign(dvec.z)});
+        float rest = M::div(1.0f * d, P.dt * vn);
         float kk = (1.0f + rest) * dot(vh, nrm);
         vh = vh - kk * nrm;
         x = cp;
@@ -138,6 +140,7 @@ __device__ __forceinline__ void accel_integrate_one(
     vel_out[i] = f4(clamp3(vt, -P.max_vel, P.max_vel), 0.0f);
 }
 
+template <bool F>
 __global__ __launch_bounds__(256) void k_accel_integrate(
     int n, const float4* __restrict__ prho, const float4* __restrict__ vel_s, const float4* __restrict__ vh_s,
     const float* __restrict__ pressure, const float4* __restrict__ normal, const int* __restrict__ nl,
@@ -146,8 +149,8 @@ __global__ __launch_bounds__(256) void k_accel_integrate(
     float4* __restrict__ pos_out, float4* __restrict__ vel_out, float4* __restrict__ vh_out) {
     int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    accel_integrate_one(i, n, prho, vel_s, vh_s, pressure, normal, nl, nc, bnl, bnc, bpos_s, bphi_s, P, accel,
-                        pos_out, vel_out, vh_out);
+    accel_integrate_one<F>(i, n, prho, vel_s, vh_s, pressure, normal, nl, nc, bnl, bnc, bpos_s, bphi_s, P, accel,
+                           pos_out, vel_out, vh_out);
 }
 
 void launch_accel_integrate(int n, const float4* prho, const float4* vel_s, const float4* vh_s,
@@ -157,8 +160,9 @@ void launch_accel_integrate(int n, const float4* prho, const float4* vel_s, cons
                             hipStream_t s, Profiler* prof) {
     if (n <= 0) return;
     PROF_BEGIN(prof, "accel_integrate", s);
-    hipLaunchKernelGGL(k_accel_integrate, dim3((n + 255) / 256), dim3(256), 0, s, n, prho, vel_s, vh_s, pressure,
-                       normal, nl, nc, bnl, bnc, bpos_s, bphi_s, P, accel, pos_out, vel_out, vh_out);
+    hipLaunchKernelGGL(P.fast ? k_accel_integrate<true> : k_accel_integrate<false>, dim3((n + 255) / 256), dim3(256),
+                       0, s, n, prho, vel_s, vh_s, pressure, normal, nl, nc, bnl, bnc, bpos_s, bphi_s, P, accel,
+                       pos_out, vel_out, vh_out);
     PROF_END(prof, s);
 }
 

@@ -40,12 +40,13 @@ extern "C" {
 /* sph_settings.flags. 0 reproduces the reference exactly. */
 #define SPH_FIX_WCSPH_DENSITY 1  /* WCSPH density evaluates W_POLY6 on r^2 instead of r (wcsph.cl:61-62,70-71) */
 /*
- * PCISPH only: the fluid kernels divide and take square roots with the GPU's
- * approximate reciprocal, square-root and reciprocal-square-root instructions
- * (about 1 ulp each), as the reference builds its kernels with
- * -cl-fast-relaxed-math (pcisphsimluation.cpp:496-497). Neighbour search, cell
- * keys and the boundary wrench are unchanged; fields differ from the default
- * IEEE build by rounding only. WCSPH, batches and the adjoint reject it.
+ * Single-scene solvers: the fluid kernels divide and take square roots with
+ * the GPU's approximate reciprocal, square-root and reciprocal-square-root
+ * instructions (about 1 ulp each), as the reference builds its kernels with
+ * -cl-fast-relaxed-math (pcisphsimluation.cpp:496-497,
+ * wcsphsimluation.cpp:364). Neighbour search, cell keys and the boundary
+ * wrench are unchanged; fields differ from the default IEEE build by rounding
+ * only. Batches (sph_batch_add_scene) and the adjoint reject it.
  */
 #define SPH_FAST_MATH 2
 

@@ -75,20 +75,19 @@ def test_last_error_is_per_call():
 
 
 def test_flag_validation():
-    """Unknown flag bits are rejected, and SPH_FAST_MATH is a PCISPH option:
-    a WCSPH scene, a batch scene and the adjoint (both WCSPH) reject it
-    before any device call."""
+    """Unknown flag bits are rejected, and SPH_FAST_MATH is a single-scene
+    option: a batch scene and the adjoint reject it before any device call."""
     L = _lib.lib()
     s = _lib.SphSettings()
     assert L.sph_default_settings(C.byref(s)) == 0
     s.method = _lib.WCSPH
     s.flags = _lib.FAST_MATH
-    assert not L.sph_create(C.byref(s))
-    assert b"PCISPH only" in L.sph_last_error()
     pos = np.zeros((1, 4), np.float32)
     p = pos.ctypes.data_as(C.POINTER(C.c_float))
     assert not L.sph_adjoint_create(C.byref(s), 1.0, 1.0, 1.0, p, p, p, 1, 1)
-    assert b"PCISPH only" in L.sph_last_error()
+    assert b"adjoint: SPH_FAST_MATH is not supported" in L.sph_last_error()
+    assert L.sph_batch_add_scene(None, C.byref(s), 1.0, 1.0, 1.0, p, 1) < 0
+    assert b"batch: SPH_FAST_MATH is not supported" in L.sph_last_error()
     s.method = _lib.PCISPH
     s.flags = 8
     assert not L.sph_create(C.byref(s))

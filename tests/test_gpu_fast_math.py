@@ -1,6 +1,8 @@
-"""SPH_FAST_MATH: the PCISPH kernels built with the GPU's approximate
+"""SPH_FAST_MATH: the single-scene kernels built with the GPU's approximate
 reciprocal, square root and reciprocal square root, as the reference builds
-its kernels with -cl-fast-relaxed-math (pcisphsimluation.cpp:496-497).
+its kernels with -cl-fast-relaxed-math (pcisphsimluation.cpp:496-497,
+wcsphsimluation.cpp:364). PCISPH is checked below at every size and over a
+long run; WCSPH one step at a time on the small scene.
 
 The bar, from the same state on both sides:
   * cell keys, sort permutation and neighbour lists bit-exact with the oracle
@@ -39,7 +41,7 @@ FAST_FIELDS = ["normal", "force_other", "force_pressure"]
 
 
 def _fast(sc):
-    sc.settings = dict(sc.settings, flags=FAST_MATH)
+    sc.settings = dict(sc.settings, flags=sc.settings.get("flags", 0) | FAST_MATH)
     return sc
 
 
@@ -85,6 +87,40 @@ def test_fast_math_single_step_1m():
     orc = build_oracle(scenes.c3_pcisph_throughput(1_000_000))
     rep = _step_and_compare(sim, orc, "1M step", sc.settings["rest_density"])
     assert sum(rep[f][1] for f in FAST_FIELDS) > 0
+
+
+WC_FIELDS = ["pos_sorted", "vel_sorted", "vel_half_sorted", "density", "pressure", "normal", "accel", "pos",
+             "vel", "vel_half"]
+
+
+def test_fast_math_wcsph_single_step():
+    sc = _fast(scenes.small_wcsph())
+    print("case fast-math wcsph small", sc.n_fluid, sc.n_boundary, flush=True)
+    sim = scenes.build(sc)
+    orc = build_oracle(scenes.small_wcsph())
+    differing = 0
+    for step in range(3):
+        pos, vel, vh = sim.positions(), sim.velocities(), sim.half_velocities()
+        orc.set_state(pos, vel, vh)
+        sim.set_positions(pos)
+        sim.set_velocities(vel)
+        sim.set_half_velocities(vh)
+        sim.step(1)
+        orc.step(1)
+        for f in INT_FIELDS:
+            a, b = sim.field(f), orc.field(f)
+            assert np.array_equal(a, b), f"wcsph step {step}: {f} differs"
+        report = {}
+        for f in WC_FIELDS:
+            a, b = sim.field(f), orc.field(f)
+            if a.ndim == 2:
+                a, b = a[:, :3], b[:, :3]
+            e = max_rel(a, b)
+            report[f] = (f"{e:.2e}", int(np.sum(a.view(np.uint32) != b.view(np.uint32))))
+            assert e <= TOL, f"wcsph step {step}: {f} max-rel {e:.3e} > {TOL}"
+        print(f"wcsph step {step} max-rel, differing words:", report, flush=True)
+        differing += report["normal"][1] + report["accel"][1]
+    assert differing > 0, "no field differs from the IEEE oracle: the fast build did not run"
 
 
 def _kinetic_and_density(sim, mass, rho0):
