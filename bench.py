@@ -219,8 +219,8 @@ def main():
 
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the solver has no multi-GPU path. "
-              "A scene would shard into Morton-contiguous slabs with halo exchanges; this build "
-              "measures one GPU.", file=sys.stderr)
+              "A scene would shard into Morton-contiguous slabs with halo exchanges; the benchmark "
+              "measures one GPU (--n 125000 runs one GPU's share of the 1M scene on 8).", file=sys.stderr)
         sys.exit(2)
 
     from fluids_amd import scenes
