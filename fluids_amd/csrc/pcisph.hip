@@ -73,7 +73,7 @@ __device__ __forceinline__ float st_kernel(float r, float h, float c) {
     return 0.0f;
 }
 
-constexpr int FORCE_BATCH = 4;
+constexpr int FORCE_BATCH = 6;
 
 template <bool F>
 __global__ __launch_bounds__(256) void k_forces_predict(int n, const float4* __restrict__ prho,

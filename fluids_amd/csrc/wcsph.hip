@@ -23,7 +23,7 @@ __device__ __forceinline__ float st_kernel_wc(float r, float h, float c) {
     return 0.0f;
 }
 
-constexpr int WC_BATCH = 4;
+constexpr int WC_BATCH = 6;
 
 // One particle of compute_acceleration + predict_vel_n_pos; n is the list
 // stride (the total particle count). F: the Math<true> build (SPH_FAST_MATH).
