@@ -793,6 +793,10 @@ void launch_lists_density(bool wcsph, int n, const float4* pos_s, const int2* fR
 
 // ------------------------------------------------------------------ normals
 // compute_normals (normals.cl:23-70); prho holds (x, y, z, density).
+// Neighbour records gathered together per thread in compute_normals (one
+// 16-byte record per neighbour; 16 measured best at 4M, equal at 1M).
+constexpr int NORMALS_BATCH = 16;
+
 template <bool F = false>
 __device__ __forceinline__ void normals_one(int i, int n, const float4* __restrict__ prho,
                                             const int* __restrict__ nl, const int* __restrict__ nc, float H2,
@@ -802,15 +806,15 @@ __device__ __forceinline__ void normals_one(int i, int n, const float4* __restri
     const int len = nc[i];
     const int* L = nl + i;
     const __amdgpu_buffer_rsrc_t rp = buffer_view(prho, (uint32_t)n * 16u);
-    for (int k0 = 0; k0 < len; k0 += GATHER_BATCH) {
-        int jj[GATHER_BATCH];
-        float4 q[GATHER_BATCH];
+    for (int k0 = 0; k0 < len; k0 += NORMALS_BATCH) {
+        int jj[NORMALS_BATCH];
+        float4 q[NORMALS_BATCH];
 #pragma unroll
-        for (int u = 0; u < GATHER_BATCH; ++u) jj[u] = k0 + u < len ? L[(size_t)(k0 + u) * n] : i;
+        for (int u = 0; u < NORMALS_BATCH; ++u) jj[u] = k0 + u < len ? L[(size_t)(k0 + u) * n] : i;
 #pragma unroll
-        for (int u = 0; u < GATHER_BATCH; ++u) q[u] = load4(rp, (uint32_t)jj[u] * 16u, 0u);
+        for (int u = 0; u < NORMALS_BATCH; ++u) q[u] = load4(rp, (uint32_t)jj[u] * 16u, 0u);
 #pragma unroll
-        for (int u = 0; u < GATHER_BATCH; ++u) {
+        for (int u = 0; u < NORMALS_BATCH; ++u) {
             if (k0 + u < len) {
                 V3 r = pi - v3(q[u]);
                 float qq = sqr(H2 - dot(r, r));
