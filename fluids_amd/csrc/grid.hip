@@ -792,11 +792,11 @@ void launch_lists_density(bool wcsph, int n, const float4* pos_s, const int2* fR
 }
 
 // ------------------------------------------------------------------ normals
-// compute_normals (normals.cl:23-70); prho holds (x, y, z, density).
 // Neighbour records gathered together per thread in compute_normals (one
 // 16-byte record per neighbour; 16 measured best at 4M, equal at 1M).
 constexpr int NORMALS_BATCH = 16;
 
+// compute_normals (normals.cl:23-70); prho holds (x, y, z, density).
 template <bool F = false>
 __device__ __forceinline__ void normals_one(int i, int n, const float4* __restrict__ prho,
                                             const int* __restrict__ nl, const int* __restrict__ nc, float H2,

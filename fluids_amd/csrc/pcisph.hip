@@ -73,6 +73,9 @@ __device__ __forceinline__ float st_kernel(float r, float h, float c) {
     return 0.0f;
 }
 
+// Neighbours whose records (position and density, velocity, normal) are
+// gathered together per thread in forces_predict (6 measured best: 4 and 8
+// were slower at 1M).
 constexpr int FORCE_BATCH = 6;
 
 template <bool F>

@@ -23,6 +23,9 @@ __device__ __forceinline__ float st_kernel_wc(float r, float h, float c) {
     return 0.0f;
 }
 
+// Neighbours whose four records are gathered together per thread in the
+// acceleration kernel (6 measured best for the batch of scenes: 4 and 8
+// were slower).
 constexpr int WC_BATCH = 6;
 
 // One particle of compute_acceleration + predict_vel_n_pos; n is the list
