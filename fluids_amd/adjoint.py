@@ -7,6 +7,11 @@ torch.autograd.Function.
     loss = ((x_T - target) ** 2).sum(1).mean()
     loss.backward()          # v0.grad, k_visc.grad, k_gas.grad
 
+A sixth argument gives the surface tension coefficient (a float, or a tensor
+that then receives its gradient); k_visc and k_gas may be plain floats, whose
+gradients are then not formed. Several rollouts may share one handle: each
+backward rebuilds its own rollout's tape if another forward ran since.
+
 v0 sets both the velocity and the half-step velocity of the initial state,
 so its gradient is the sum of the two. The forward pass is the solver's
 (bit for bit the same trajectory as sph_step); the backward pass runs on the
@@ -73,6 +78,9 @@ class WCSPHAdjoint:
 
     def forward(self, steps):
         check(lib().sph_adjoint_forward(self._h, int(steps)))
+        # the handle holds one tape: a rollout whose backward comes after
+        # another forward on the same handle must run its own forward again
+        self.generation = getattr(self, "generation", 0) + 1
 
     def final_state(self):
         out = [np.zeros((self.n, 4), np.float32) for _ in range(3)]
@@ -105,22 +113,35 @@ def _torch_function():
         def forward(ctx, v0, k_visc, k_gas, adj, steps, surface_tension=None):
             v = v0.detach().cpu().numpy().astype(np.float32)
             sigma = adj.settings["surface_tension"] if surface_tension is None else float(surface_tension)
+            params = (float(k_visc), float(k_gas), sigma)
             adj.set_initial(None, v, v)
-            adj.set_params(float(k_visc), float(k_gas), sigma)
+            adj.set_params(*params)
             adj.forward(int(steps))
             pos, _, _ = adj.final_state()
             ctx.adj = adj
-            ctx.meta = (v0.dtype, v0.device, k_visc.dtype if torch.is_tensor(k_visc) else torch.float64)
+            ctx.replay = (v, params, int(steps), adj.generation)
+            ctx.meta = (v0.dtype, v0.device)
+            ctx.pmeta = tuple((a.dtype, a.device) if torch.is_tensor(a) else None
+                              for a in (k_visc, k_gas, surface_tension))
             return torch.from_numpy(pos[:, :3].copy()).to(device=v0.device, dtype=v0.dtype)
 
         @staticmethod
         def backward(ctx, g_pos):
-            g = ctx.adj.backward(g_pos.detach().cpu().numpy().astype(np.float32))
-            dtype, device, pdtype = ctx.meta
+            adj = ctx.adj
+            v, params, steps, gen = ctx.replay
+            if adj.generation != gen:
+                # another rollout ran forward on this handle since: rebuild
+                # this rollout's tape (same inputs, same bits) first
+                adj.set_initial(None, v, v)
+                adj.set_params(*params)
+                adj.forward(steps)
+            g = adj.backward(g_pos.detach().cpu().numpy().astype(np.float32))
+            dtype, device = ctx.meta
             g_v0 = torch.from_numpy((g["vel0"] + g["vh0"]).astype(np.float32)).to(device=device, dtype=dtype)
-            g_kv = torch.tensor(g["k_viscosity"], dtype=pdtype, device=device)
-            g_kg = torch.tensor(g["gas_stiffness"], dtype=pdtype, device=device)
-            return g_v0, g_kv, g_kg, None, None, None
+            out = []
+            for key, meta in zip(("k_viscosity", "gas_stiffness", "surface_tension"), ctx.pmeta):
+                out.append(None if meta is None else torch.tensor(g[key], dtype=meta[0], device=meta[1]))
+            return g_v0, out[0], out[1], None, None, out[2]
 
     return WCSPHRollout
 

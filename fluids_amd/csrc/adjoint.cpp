@@ -166,6 +166,11 @@ void WcAdjoint::forward(int nsteps) {
         if (t % every_ == 0) save_checkpoint(t / every_);
         run_step(t % every_, W);
     }
+    // the particle ids of the final state, by stored index: d_ids_a is
+    // overwritten when backward recomputes an earlier segment
+    final_ids_.resize(n_);
+    hip_check(hipMemcpyAsync(final_ids_.data(), d_ids_a.get(), n_ * sizeof(int), hipMemcpyDeviceToHost, stream_),
+              "download");
     hip_check(hipStreamSynchronize(stream_), "sync");
     forward_ms_ = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     steps_ = nsteps;
@@ -198,10 +203,10 @@ void WcAdjoint::backward(const float* gx, const float* gv, const float* gvh, flo
     if (steps_ == 0) throw Error(SPH_ERR_STATE, "adjoint: run forward first");
     auto t0 = std::chrono::steady_clock::now();
     const WcStepParams W = params();
-    // adjoint of the final state, by stored index
-    std::vector<int> ids(n_);
-    hip_check(hipMemcpyAsync(ids.data(), d_ids_a.get(), n_ * sizeof(int), hipMemcpyDeviceToHost, stream_), "download");
-    hip_check(hipStreamSynchronize(stream_), "sync");
+    // adjoint of the final state, by stored index (the ids kept by forward,
+    // so that a repeated backward maps the cotangent the same way)
+    const std::vector<int>& ids = final_ids_;
+    if ((int)ids.size() != n_) throw Error(SPH_ERR_STATE, "adjoint: run forward first");
     std::vector<float4> g(3 * (size_t)n_, make_float4(0, 0, 0, 0));
     for (int i = 0; i < n_; ++i) {
         int id = ids[i];

@@ -62,6 +62,7 @@ private:
     Grid grid_;
     hipStream_t stream_ = nullptr;
     std::vector<float4> init_pos_, init_vel_, init_vh_;
+    std::vector<int> final_ids_;  // particle id at each stored index after forward
     std::vector<std::unique_ptr<Slot>> tape_;
     std::vector<std::unique_ptr<Checkpoint>> ck_;
 

@@ -118,6 +118,56 @@ def test_adjoint_matches_torch_autograd(which):
         assert err < 2e-3, (which, name, err)
 
 
+def _small_case():
+    case = scenes.c5_adjoint(nx=8, ny=8, nz=6, steps=12)
+    sc = case.scene
+    v4 = np.concatenate([case.v0_hidden, np.zeros((sc.n_fluid, 1), np.float32)], 1)
+    return case, sc, v4
+
+
+def test_adjoint_backward_twice_gives_the_same_gradients():
+    """Three checkpoint segments: the first backward leaves the tape and the
+    stored arrays at segment 0; a second backward must map the cotangent onto
+    the same particles and give identical gradients."""
+    case, sc, v4 = _small_case()
+    n = sc.n_fluid
+    adj = WCSPHAdjoint(sc.settings, sc.container, sc.volumes[0], v4, v4, checkpoint_every=5)
+    adj.forward(12)
+    rng = np.random.default_rng(3)
+    gx, gv = rng.standard_normal((2, n, 3)).astype(np.float32)
+    a = adj.backward(gx, gv)
+    b = adj.backward(gx, gv)
+    for k in ("pos0", "vel0", "vh0"):
+        assert np.array_equal(a[k], b[k]), k
+        assert np.abs(a[k]).max() > 0
+    for k in ("k_viscosity", "gas_stiffness", "surface_tension"):
+        assert a[k] == b[k], k
+
+
+def test_rollout_function_two_applies_on_one_handle():
+    """Two rollouts through one handle before a single backward: each gets
+    its own gradient, equal to that of a rollout run alone; plain-float
+    parameters get no gradient and a tensor surface tension gets one."""
+    from fluids_amd.adjoint import WCSPHRollout
+    case, sc, v4 = _small_case()
+    st = sc.settings
+    adj = WCSPHAdjoint(st, sc.container, sc.volumes[0], v4, v4, checkpoint_every=5)
+    va = torch.tensor(case.v0_hidden, requires_grad=True)
+    vb = torch.tensor(0.5 * case.v0_hidden + 0.1, requires_grad=True)
+    kv = torch.tensor(float(st["k_viscosity"]), dtype=torch.float64, requires_grad=True)
+    xa = WCSPHRollout.apply(va, kv, float(st["gas_stiffness"]), adj, 12)
+    xb = WCSPHRollout.apply(vb, kv, float(st["gas_stiffness"]), adj, 12)
+    (xa.sum() + (xb ** 2).sum()).backward()
+    alone = torch.tensor(case.v0_hidden, requires_grad=True)
+    sg = torch.tensor(float(st["surface_tension"]), dtype=torch.float64, requires_grad=True)
+    WCSPHRollout.apply(alone, float(st["k_viscosity"]), float(st["gas_stiffness"]), adj, 12, sg).sum().backward()
+    assert torch.equal(va.grad, alone.grad)
+    assert kv.grad is not None and sg.grad is not None and float(sg.grad) != 0.0
+    alone_b = torch.tensor(0.5 * case.v0_hidden + 0.1, requires_grad=True)
+    (WCSPHRollout.apply(alone_b, float(st["k_viscosity"]), float(st["gas_stiffness"]), adj, 12) ** 2).sum().backward()
+    assert torch.equal(vb.grad, alone_b.grad)
+
+
 @pytest.fixture(scope="module")
 def c5_gradient():
     """Gradient of the C5 loss at a point away from the optimum."""
