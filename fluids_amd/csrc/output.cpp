@@ -58,28 +58,40 @@ void FrameWriter::write() {
         slots_[k].busy = true;
     }
     Slot& s = slots_[k];
-    const size_t need = (size_t)n * per;
-    if (s.cap < need) {
-        if (s.host) hip_check(hipHostFree(s.host), "hipHostFree");
-        s.host = nullptr;
-        hip_check(hipHostMalloc((void**)&s.host, std::max<size_t>(need, 1) * sizeof(float4), hipHostMallocDefault),
-                  "hipHostMalloc");
-        s.cap = need;
+    try {
+        const size_t need = (size_t)n * per;
+        if (s.cap < need) {
+            if (s.host) hip_check(hipHostFree(s.host), "hipHostFree");
+            s.host = nullptr;
+            s.cap = 0;
+            hip_check(hipHostMalloc((void**)&s.host, std::max<size_t>(need, 1) * sizeof(float4), hipHostMallocDefault),
+                      "hipHostMalloc");
+            s.cap = need;
+        }
+        s.n = n;
+        s.frame = (unsigned long long)sim_.frames_done;
+        s.time = sim_.sim_time;
+        const float4* pos = sim_.d_pos.get();
+        const float4* vel = sim_.d_vel.get();
+        hipStream_t st = sim_.stream();
+        size_t off = 0;
+        if (n > 0 && (fields_ & 1)) {
+            hip_check(hipMemcpyAsync(s.host, pos, n * sizeof(float4), hipMemcpyDeviceToHost, st), "frames copy");
+            off = n;
+        }
+        if (n > 0 && (fields_ & 2))
+            hip_check(hipMemcpyAsync(s.host + off, vel, n * sizeof(float4), hipMemcpyDeviceToHost, st), "frames copy");
+        hip_check(hipEventRecord(s.done, st), "hipEventRecord");
+    } catch (...) {
+        // the slot was never queued: give it back so that later writes do
+        // not wait for it
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            s.busy = false;
+        }
+        cv_.notify_all();
+        throw;
     }
-    s.n = n;
-    s.frame = (unsigned long long)sim_.frames_done;
-    s.time = sim_.sim_time;
-    const float4* pos = sim_.d_pos.get();
-    const float4* vel = sim_.d_vel.get();
-    hipStream_t st = sim_.stream();
-    size_t off = 0;
-    if (n > 0 && (fields_ & 1)) {
-        hip_check(hipMemcpyAsync(s.host, pos, n * sizeof(float4), hipMemcpyDeviceToHost, st), "frames copy");
-        off = n;
-    }
-    if (n > 0 && (fields_ & 2))
-        hip_check(hipMemcpyAsync(s.host + off, vel, n * sizeof(float4), hipMemcpyDeviceToHost, st), "frames copy");
-    hip_check(hipEventRecord(s.done, st), "hipEventRecord");
     {
         std::lock_guard<std::mutex> lk(m_);
         queue_.push_back(k);
