@@ -261,7 +261,8 @@ static void radix_pass(const uint32_t* kin, const int* vin, uint32_t* kout, int*
 int sort_passes(int key_bits, int sid_bits) { return sort_passes(key_bits) + (sid_bits > 0 ? 1 : 0); }
 
 void launch_radix_sort(uint32_t* keys_a, uint32_t* keys_b, int* vals_a, int* vals_b, uint32_t* hist, int n,
-                       int key_bits, hipStream_t s, Profiler* prof, const int* sid, int sid_bits) {
+                       int key_bits, hipStream_t s, Profiler* prof, const int* sid, int sid_bits,
+                       const int* vals_in) {
     if (n <= 0) return;
     int passes;
     const int bits = digit_bits(key_bits, passes);
@@ -273,7 +274,7 @@ void launch_radix_sort(uint32_t* keys_a, uint32_t* keys_b, int* vals_a, int* val
     // even; either way the result lands in (keys_a, vals_a).
     uint32_t* kin = (total % 2) ? keys_b : keys_a;
     uint32_t* kout = (total % 2) ? keys_a : keys_b;
-    int* vin = nullptr;
+    const int* vin = vals_in;
     int* vout = (total % 2) ? vals_a : vals_b;
     for (int p = 0; p < total; ++p) {
         const bool scene_pass = sid && p == passes;
@@ -856,7 +857,6 @@ __global__ void k_step_begin(DevState* st) {
         st->out_of_grid = 0;
         st->edge_reads = 0;
         st->nonfinite = 0;
-        st->bar_arrive = 0u;
     }
 }
 

@@ -13,9 +13,12 @@
 //                        predict_vel_n_pos (pcisph.cl:240-284)
 // The exit test (pcisphsimluation.cpp:142-147) is evaluated on the device by
 // every thread of pressure_force from the reduced maximum, so the loop runs
-// without a host round trip. The corrections that always run get one launch
-// per phase; the rest of the loop is one launch (correction_tail) that returns
-// at once when the loop has ended.
+// without a host round trip. Every correction gets one launch per phase; those
+// past the ones that always run use small grid-stride grids and return at once
+// when the loop has already ended (the usual case with the reference's
+// parameters). A loop cap above a few corrections (the relaxation step's
+// 10000) is enqueued in chunks, with the host reading the exit flag between
+// chunks.
 #include "kernels.h"
 #include "solver.h"
 #include "sph_device.h"
@@ -336,34 +339,6 @@ __global__ __launch_bounds__(256) void k_pressure_force(
                             pos_out, vel_out, st);
 }
 
-// Corrections min_iter + 1 .. max_iter in one launch. They run only when the
-// exit test failed after min_iter corrections, which the reference's
-// parameters make rare, so the usual cost is one launch that returns at once
-// (instead of two per possible correction). When they do run, every block
-// of the grid takes part in each phase and the phases are separated by grid
-// barriers; the launcher keeps the grid within what the GPU holds resident at
-// once. The exit decision is read after a barrier, so all blocks leave the
-// loop at the same correction; a barrier that times out ends the launch and
-// is reported by the step statistics.
-template <bool F>
-__global__ __launch_bounds__(256) void k_correction_tail(
-    int n, int first, const float4* __restrict__ prho, const float4* __restrict__ vel_s,
-    const float4* __restrict__ f_other, const int* __restrict__ nl, const int* __restrict__ nc,
-    const int* __restrict__ bnl, const int* __restrict__ bnc, const float4* __restrict__ bpos_s,
-    const float* __restrict__ bphi_s, int has_b, PciParams P, float* err, float* pressure, float4* pc,
-    float4* f_press, float4* pos_pred, float4* pos_out, float4* vel_out, DevState* st) {
-    for (int iter = first; iter <= P.max_iter; ++iter) {
-        const int done = __hip_atomic_load(&st->done_iter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (done != 0 && done < iter) return;
-        update_pressure_phase<F>(n, iter, pos_pred, prho, nl, nc, bnl, bnc, bpos_s, bphi_s, has_b, P, err, pressure,
-                                 pc, st);
-        if (!grid_barrier(st, P.barrier_ticks)) return;
-        pressure_force_phase<F>(n, iter, pc, vel_s, f_other, nl, nc, bnl, bnc, bpos_s, bphi_s, has_b, P, f_press,
-                                pos_pred, pos_out, vel_out, st);
-        if (!grid_barrier(st, P.barrier_ticks)) return;
-    }
-}
-
 // Final integration with a zero pressure force, for a loop of zero corrections.
 template <bool F>
 __global__ __launch_bounds__(256) void k_integrate_only(int n, const float4* __restrict__ pos_s,
@@ -383,27 +358,10 @@ __global__ __launch_bounds__(256) void k_integrate_only(int n, const float4* __r
 // later ones run in k_correction_tail.
 static int correction_blocks(int n) { return (n + 255) / 256; }
 
-// Upper bound of the tail grid (two workgroups per CU of a 256-CU device).
-constexpr int TAIL_MAX_BLOCKS = 512;
-
-// Blocks of k_correction_tail (the IEEE or the fast build) that the current
-// device holds resident at once (cached per device), so that its grid
-// barriers can complete.
-static int resident_tail_blocks(bool fast) {
-    static int cached[64][2] = {};
-    int dev = 0;
-    hip_check(hipGetDevice(&dev), "hipGetDevice");
-    if (dev >= 0 && dev < 64 && cached[dev][fast] > 0) return cached[dev][fast];
-    int per_cu = 0, cus = 0;
-    hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                  &per_cu, fast ? k_correction_tail<true> : k_correction_tail<false>, 256, 0),
-              "occupancy");
-    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "device attribute");
-    const int r = per_cu * cus;
-    if (r < 1) throw Error(SPH_ERR_HIP, "k_correction_tail cannot be resident on this device");
-    if (dev >= 0 && dev < 64) cached[dev][fast] = r;
-    return r;
-}
+// Grid of the launches for corrections past the ones that always run: they
+// usually return at once (the exit test ended the loop), so a small
+// grid-stride grid keeps an idle launch cheap.
+constexpr int TAIL_BLOCKS = 256;
 
 void launch_forces_predict(int n, const float4* prho, const float4* vel_s, const float4* normal, const int* nl,
                            const int* nc, const PciParams& P, float4* f_other, float4* pos_pred, hipStream_t s,
@@ -439,20 +397,25 @@ void launch_pressure_force(int n, int iter, const float4* pc, const float4* vel_
     PROF_END(prof, s);
 }
 
-void launch_correction_tail(int n, int first, const float4* prho, const float4* vel_s, const float4* f_other,
-                            const int* nl, const int* nc, const int* bnl, const int* bnc, const float4* bpos_s,
-                            const float* bphi_s, int has_b, const PciParams& P, float* err, float* pressure,
-                            float4* pc, float4* f_press, float4* pos_pred, float4* pos_out, float4* vel_out,
-                            DevState* st, hipStream_t s, Profiler* prof) {
-    if (n <= 0 || first > P.max_iter) return;
-    int blocks = correction_blocks(n);
-    const int resident = resident_tail_blocks(P.fast != 0);
-    const int cap = TAIL_MAX_BLOCKS < resident ? TAIL_MAX_BLOCKS : resident;
-    if (blocks > cap) blocks = cap;
-    PROF_BEGIN(prof, "correction_tail", s);
-    hipLaunchKernelGGL(P.fast ? k_correction_tail<true> : k_correction_tail<false>, dim3(blocks), dim3(256), 0, s, n, first, prho, vel_s, f_other, nl, nc, bnl,
-                       bnc, bpos_s, bphi_s, has_b, P, err, pressure, pc, f_press, pos_pred, pos_out, vel_out, st);
-    PROF_END(prof, s);
+void launch_corrections(int n, int first, int last, const float4* prho, const float4* vel_s, const float4* f_other,
+                        const int* nl, const int* nc, const int* bnl, const int* bnc, const float4* bpos_s,
+                        const float* bphi_s, int has_b, const PciParams& P, float* err, float* pressure,
+                        float4* pc, float4* f_press, float4* pos_pred, float4* pos_out, float4* vel_out,
+                        DevState* st, hipStream_t s, Profiler* prof) {
+    if (n <= 0) return;
+    const int blocks = correction_blocks(n) < TAIL_BLOCKS ? correction_blocks(n) : TAIL_BLOCKS;
+    for (int iter = first; iter <= last; ++iter) {
+        PROF_BEGIN(prof, "update_pressure_tail", s);
+        hipLaunchKernelGGL(P.fast ? k_update_pressure<true> : k_update_pressure<false>, dim3(blocks), dim3(256), 0,
+                           s, n, iter, pos_pred, prho, nl, nc, bnl, bnc, bpos_s, bphi_s, has_b, P, err, pressure, pc,
+                           st);
+        PROF_END(prof, s);
+        PROF_BEGIN(prof, "pressure_force_tail", s);
+        hipLaunchKernelGGL(P.fast ? k_pressure_force<true> : k_pressure_force<false>, dim3(blocks), dim3(256), 0, s,
+                           n, iter, pc, vel_s, f_other, nl, nc, bnl, bnc, bpos_s, bphi_s, has_b, P, f_press, pos_pred,
+                           pos_out, vel_out, st);
+        PROF_END(prof, s);
+    }
 }
 
 void launch_pci_integrate_only(int n, const float4* pos_s, const float4* vel_s, const float4* f_other,

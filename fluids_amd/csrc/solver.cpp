@@ -731,8 +731,6 @@ void SimulationBase::stats(sph_step_stats& o) {
     launch_stats(n_, d_nc.get(), d_bnc.get(), d_pos.get(), d_acc.get(), stream_);
     hip_check(hipMemcpyAsync(acc, d_acc.get(), sizeof acc, hipMemcpyDeviceToHost, stream_), "download");
     hip_check(hipStreamSynchronize(stream_), "sync");
-    if (st.bar_timeouts != 0)
-        throw Error(SPH_ERR_HIP, "PCISPH correction loop: a grid barrier timed out; the step's results are invalid");
     o.iterations = st.iterations;
     o.edge_reads = st.edge_reads;
     o.out_of_grid = st.out_of_grid;
@@ -749,18 +747,6 @@ void SimulationBase::stats(sph_step_stats& o) {
 }
 
 // ----------------------------------------------------------------- PCISPH
-// Frequency of the device wall clock that kernels read (cached per device).
-static double device_wall_clock_hz() {
-    static int cached_khz[64] = {0};
-    int dev = 0;
-    hip_check(hipGetDevice(&dev), "hipGetDevice");
-    if (dev >= 0 && dev < 64 && cached_khz[dev] > 0) return 1000.0 * cached_khz[dev];
-    int khz = 0;
-    hip_check(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev), "device attribute");
-    if (khz <= 0) throw Error(SPH_ERR_HIP, "device reports no wall-clock rate");
-    if (dev >= 0 && dev < 64) cached_khz[dev] = khz;
-    return 1000.0 * khz;
-}
 
 PCISPHSimulation::PCISPHSimulation(const Settings& s) : SimulationBase(s) { initialize_params(); }
 
@@ -841,9 +827,6 @@ void PCISPHSimulation::fill_params() {
     P.min_iter = 3;
     P.max_iter = s_.pcisph_max_iterations;
     P.fast = (s_.flags & SPH_FAST_MATH) ? 1 : 0;
-    P.barrier_ticks = barrier_timeout_s_ > 0.0
-                          ? (unsigned long long)(barrier_timeout_s_ * device_wall_clock_hz())
-                          : 0ull;
 }
 
 // relaxation step and velocity reset (pcisphsimluation.cpp:548-550)
@@ -891,9 +874,12 @@ void PCISPHSimulation::simulate_step(int min_iter, int max_iter) {
     }
     launch_forces_predict(n_, d_prho.get(), d_vel_s.get(), d_normal.get(), d_nl.get(), d_nc.get(), P_,
                           d_f_other.get(), d_pos_pred.get(), s, &prof_);
-    // The corrections that always run get one launch per phase; the rest of
-    // the loop, up to max_iter, is one launch that ends at the device-side
-    // exit test, so the step needs no host round trip.
+    // The corrections that always run get one launch per phase on full grids;
+    // the rest of the loop, up to max_iter, on small grids that return at once
+    // after the device-side exit test ended the loop, so a step needs no host
+    // round trip. A cap above TAIL_CHUNK further corrections (the relaxation
+    // step) is enqueued TAIL_CHUNK corrections at a time, and the host reads
+    // the exit flag between chunks.
     const int always = min_iter < max_iter ? min_iter : max_iter;
     for (int i = 1; i <= always; ++i) {
         launch_update_pressure(n_, i, d_pos_pred.get(), d_prho.get(), d_nl.get(), d_nc.get(), d_bnl.get(),
@@ -903,10 +889,21 @@ void PCISPHSimulation::simulate_step(int min_iter, int max_iter) {
                               d_bnc.get(), boundary_.d_pos_s.get(), boundary_.d_phi_s.get(), has_b, P_,
                               d_f_press.get(), d_pos_pred.get(), d_pos.get(), d_vel.get(), d_state.get(), s, &prof_);
     }
-    launch_correction_tail(n_, always + 1, d_prho.get(), d_vel_s.get(), d_f_other.get(), d_nl.get(), d_nc.get(),
+    constexpr int TAIL_CHUNK = 8;
+    for (int first = always + 1; first <= max_iter; first += TAIL_CHUNK) {
+        if (first > always + 1) {
+            int done = 0;
+            hip_check(hipMemcpyAsync(&done, &d_state.get()->done_iter, sizeof done, hipMemcpyDeviceToHost, s),
+                      "download");
+            hip_check(hipStreamSynchronize(s), "sync");
+            if (done != 0) break;
+        }
+        const int last = std::min(max_iter, first + TAIL_CHUNK - 1);
+        launch_corrections(n_, first, last, d_prho.get(), d_vel_s.get(), d_f_other.get(), d_nl.get(), d_nc.get(),
                            d_bnl.get(), d_bnc.get(), boundary_.d_pos_s.get(), boundary_.d_phi_s.get(), has_b, P_,
                            d_err.get(), d_pressure.get(), d_pc.get(), d_f_press.get(), d_pos_pred.get(), d_pos.get(),
                            d_vel.get(), d_state.get(), s, &prof_);
+    }
     boundary_.apply_fluid_forces();
 }
 

@@ -261,9 +261,6 @@ public:
     }
     float density_scale_factor() const { return delta_; }
     PciParams params() const { return P_; }
-    // Bound of each grid-barrier wait in the correction loop (seconds; <= 0
-    // gives up at once, which only failure-handling tests use).
-    void set_barrier_timeout(double seconds) { barrier_timeout_s_ = seconds; }
 
 protected:
     void initialize_params() override;
@@ -272,7 +269,6 @@ protected:
     void deduce_density_scale_factor();
     void fill_params();
     float mass_ = 0.0f, st_main_ = 0.0f, st_term_ = 0.0f, delta_ = 0.0f;
-    double barrier_timeout_s_ = 5.0;
     PciParams P_{};
 };
 

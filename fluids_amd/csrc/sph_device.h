@@ -75,9 +75,6 @@ struct DevState {
     int out_of_grid;
     int edge_reads;
     int nonfinite;                      // non-finite positions met by the hash kernel
-    unsigned int bar_arrive;            // grid barrier: blocks arrived in the current round
-    unsigned int bar_gen;               // grid barrier: completed rounds
-    int bar_timeouts;                   // grid barrier waits that gave up (sticky)
 };
 
 __device__ __forceinline__ float cl_max(float a, float b) { return (a < b) ? b : a; }
@@ -279,44 +276,6 @@ __device__ __forceinline__ void list_batch(const int* __restrict__ list, size_t 
         const int j = list[(size_t)k * stride + i];
         jj[u] = k0 + u < len ? j : dflt;
     }
-}
-
-// Grid-wide barrier for a launch whose workgroups are all resident at once.
-// Thread 0 of each workgroup makes the workgroup's writes visible device-wide
-// (release fence), counts its arrival, and waits until the last arrival
-// advances the round; the wait polls with relaxed loads (an acquiring poll
-// would invalidate the caches on every read) and the fence after it makes
-// every other workgroup's writes visible. The wait is bounded by `timeout` ticks of the device wall
-// clock: a workgroup that gives up counts a timeout and returns false, and
-// its caller leaves the kernel, so no wave waits forever.
-__device__ __forceinline__ bool grid_barrier(DevState* st, unsigned long long timeout) {
-    __shared__ int ok;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned int gen = __hip_atomic_load(&st->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned int arrived =
-            __hip_atomic_fetch_add(&st->bar_arrive, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        int good = 1;
-        if (arrived == gridDim.x - 1) {
-            __hip_atomic_store(&st->bar_arrive, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(&st->bar_gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            const unsigned long long t0 = wall_clock64();
-            while (__hip_atomic_load(&st->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-                if (wall_clock64() - t0 > timeout) {
-                    __hip_atomic_fetch_add(&st->bar_timeouts, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    good = 0;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(2);
-            }
-        }
-        __threadfence();
-        ok = good;
-    }
-    __syncthreads();
-    return ok != 0;
 }
 
 // Sum over the 64 lanes of a wave by butterfly shuffles (lane 0's value is

@@ -223,9 +223,6 @@ class FluidSimulation:
         check(lib().sph_get_step_stats(self._h, C.byref(s)))
         return {k: getattr(s, k) for k, _ in SphStepStats._fields_}
 
-    def set_barrier_timeout(self, seconds: float):
-        """Bound of each grid-barrier wait of the PCISPH correction loop."""
-        check(lib().sph_set_barrier_timeout(self._h, float(seconds)))
 
     def set_profiling(self, on=True, kernel: str = None):
         """HIP-event timing of every launch, or of one kernel's launches only."""
