@@ -7,7 +7,9 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libsph.so")
+# SPH_LIB names another build of the library (the host-sanitizer build of
+# tests/test_sanitizers.py); by default the in-tree libsph.so
+LIB_PATH = os.environ.get("SPH_LIB") or os.path.join(HERE, "libsph.so")
 
 SPH_OK = 0
 WCSPH, PCISPH = 0, 1
@@ -84,11 +86,11 @@ SIGNATURES = {
     "sph_get_step_stats": (I, [P, C.POINTER(SphStepStats)]),
     "sph_device_state": (I, [P, P, P, P]),
     "sph_stream": (P, [P]),
+    "sph_sort_pairs": (I, [P, P, I, I, P]),
     "sph_set_profiling": (I, [P, I]),
     "sph_set_profiling_filter": (I, [P, C.c_char_p]),
     "sph_get_kernel_times": (I, [P, P, P, P, I]),
     "sph_reset_kernel_times": (I, [P]),
-    "sph_set_barrier_timeout": (I, [P, C.c_double]),
     "sph_sample_box_volume": (I, [P, P, F, P, I]),
     "sph_sample_fishtank": (I, [F, F, F, F, P, I]),
     "sph_sample_cube": (I, [F, F, P, I]),

@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsph.so")
 
-SOURCES = ["grid.hip", "pcisph.hip", "wcsph.hip", "boundary.hip", "adjoint.hip", "adjoint.cpp", "solver.cpp", "mesh_samplers.cpp", "scene.cpp", "output.cpp", "rigid.cpp", "batch.cpp", "sph_abi.cpp"]
+SOURCES = ["grid.hip", "sort.hip", "pcisph.hip", "wcsph.hip", "boundary.hip", "adjoint.hip", "adjoint.cpp", "solver.cpp", "mesh_samplers.cpp", "scene.cpp", "output.cpp", "rigid.cpp", "batch.cpp", "sph_abi.cpp"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
@@ -55,6 +55,38 @@ def build(verbose: bool = False, jobs: int = 6) -> str:
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     return LIB
+
+
+# Host C++ sources (no kernels); the sanitizer build instruments these only.
+HOST_SOURCES = [f for f in SOURCES if f.endswith(".cpp")]
+
+
+def build_host_sanitized(out_dir: str) -> str:
+    """libsph with AddressSanitizer and UndefinedBehaviorSanitizer on the host
+    C++ sources (solver, samplers, scene files, rigid world, output, batch
+    and adjoint drivers, the C ABI) into out_dir; the HIP sources are linked
+    from the regular build, so their device code is unchanged. Load it with
+    clang's ASan runtime preloaded."""
+    build()
+    os.makedirs(out_dir, exist_ok=True)
+    san = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+           "-Xarch_host", "-fno-omit-frame-pointer"]
+    flags = [f for f in COMMON if f != "-O3"] + ["-O1", "-g"]
+
+    def one(src):
+        out = os.path.join(out_dir, src + ".o")
+        r = subprocess.run(["hipcc", "-x", "hip", *flags, *san, "-c", os.path.join(CSRC, src), "-o", out],
+                           capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"hipcc failed on {src}:\n{r.stderr}")
+        return out
+    with cf.ThreadPoolExecutor(max_workers=6) as ex:
+        objs = list(ex.map(one, HOST_SOURCES))
+    objs += [os.path.join(OBJ, f + ".o") for f in SOURCES if f.endswith(".hip")]
+    lib = os.path.join(out_dir, "libsph_asan.so")
+    subprocess.run(["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", *objs, *san, "-Xarch_host",
+                    "-shared-libasan", "-o", lib], check=True, capture_output=True)
+    return lib
 
 
 if __name__ == "__main__":

@@ -73,7 +73,6 @@ struct PciParams {
     float lo_x, hi_x, lo_y, lo_z, hi_z;  // container clamp: (-c)/2 and c/2; y is clamped to [lo_y, 10]
     float rho0_host, eta;                // exit test operands (host floats)
     int min_iter, max_iter;
-    unsigned long long barrier_ticks;    // bound of each grid-barrier wait, device wall-clock ticks
     int fast;                            // 1: Math<true> builds of the kernels (SPH_FAST_MATH)
 };
 
@@ -94,8 +93,11 @@ void launch_hash(int n, const float4* pos, const GridDev& g, uint32_t* keys, con
 // Stable sort of n (key, index) pairs by the low key_bits of the keys; with
 // sid (scene id of each index, < 2^sid_bits) a final pass orders the result
 // by scene, so each scene's contiguous segment is sorted by its own keys.
+// vals_in (optional) holds the values to carry; without it they are the
+// indices 0 .. n-1.
 void launch_radix_sort(uint32_t* keys_a, uint32_t* keys_b, int* vals_a, int* vals_b, uint32_t* hist, int n,
-                       int key_bits, hipStream_t s, Profiler* prof, const int* sid = nullptr, int sid_bits = 0);
+                       int key_bits, hipStream_t s, Profiler* prof, const int* sid = nullptr, int sid_bits = 0,
+                       const int* vals_in = nullptr);
 void launch_gather_fluid(int n, const int* perm, const uint32_t* keys_s, const float4* pos, const float4* vel,
                          const float4* vh, float4* pos_s, float4* vel_s, float4* vh_s, int2* R, const GridDev& g,
                          hipStream_t s, Profiler* prof);
@@ -119,6 +121,9 @@ void launch_step_begin(DevState* st, hipStream_t s);
 void launch_fill_u32(void* p, size_t n, uint32_t v, hipStream_t s);
 void launch_stats(int n, const int* nc, const int* bnc, const float4* pos, unsigned long long* acc, hipStream_t s);
 
+// sort.hip: the body of sph_sort_pairs (throws sph::Error)
+void sort_pairs(uint32_t* keys, int32_t* values, int n, int key_bits, void* stream);
+
 // pcisph.hip
 void launch_forces_predict(int n, const float4* prho, const float4* vel_s, const float4* normal, const int* nl,
                            const int* nc, const PciParams& P, float4* f_other, float4* pos_pred, hipStream_t s,
@@ -131,13 +136,13 @@ void launch_pressure_force(int n, int iter, const float4* pc, const float4* vel_
                            const int* nl, const int* nc, const int* bnl, const int* bnc, const float4* bpos_s,
                            const float* bphi_s, int has_b, const PciParams& P, float4* f_press, float4* pos_pred,
                            float4* pos_out, float4* vel_out, DevState* st, hipStream_t s, Profiler* prof);
-// Corrections first .. P.max_iter in one launch; returns at once when the
-// loop already ended.
-void launch_correction_tail(int n, int first, const float4* prho, const float4* vel_s, const float4* f_other,
-                            const int* nl, const int* nc, const int* bnl, const int* bnc, const float4* bpos_s,
-                            const float* bphi_s, int has_b, const PciParams& P, float* err, float* pressure,
-                            float4* pc, float4* f_press, float4* pos_pred, float4* pos_out, float4* vel_out,
-                            DevState* st, hipStream_t s, Profiler* prof);
+// Corrections first .. last, one launch per phase on a grid-stride grid of at
+// most 256 workgroups; each returns at once when the loop already ended.
+void launch_corrections(int n, int first, int last, const float4* prho, const float4* vel_s, const float4* f_other,
+                        const int* nl, const int* nc, const int* bnl, const int* bnc, const float4* bpos_s,
+                        const float* bphi_s, int has_b, const PciParams& P, float* err, float* pressure,
+                        float4* pc, float4* f_press, float4* pos_pred, float4* pos_out, float4* vel_out,
+                        DevState* st, hipStream_t s, Profiler* prof);
 void launch_pci_integrate_only(int n, const float4* pos_s, const float4* vel_s, const float4* f_other,
                                const PciParams& P, float4* pos_out, float4* vel_out, hipStream_t s,
                                Profiler* prof);

@@ -1,0 +1,66 @@
+// sph_sort_pairs: the library's stable key/value radix sort on device arrays
+// the caller owns, in place of the reference's clsort (opencl/algorithms/
+// clsort.h:12-35, a clogs::Radixsort over cl_mem keys and values), with the
+// kernels of grid.hip. Its ping-pong arrays and digit histograms come from
+// the stream-ordered allocator.
+#include <string>
+
+#include "kernels.h"
+#include "solver.h"
+#include "sph.h"
+
+namespace sph {
+
+// Device memory one sort of n pairs needs beyond its inputs: the other half
+// of the key and value ping-pong and the digit histograms, each 256-byte
+// aligned.
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+static size_t sort_scratch_bytes(int n) {
+    return 2 * align256((size_t)n * 4) + align256(sort_hist_words(n) * 4);
+}
+
+static void check_args(const uint32_t* keys, const int32_t* values, int n, int& key_bits) {
+    if (n < 0) throw Error(SPH_ERR_INVALID, "sort: negative size");
+    if (key_bits == 0) key_bits = 32;
+    if (key_bits < 1 || key_bits > 32) throw Error(SPH_ERR_INVALID, "sort: key_bits must be 0 or 1..32");
+    if (n > 0 && (!keys || !values)) throw Error(SPH_ERR_INVALID, "sort: keys and values are required");
+    if (n > MAX_SCENE_PARTICLES) throw Error(SPH_ERR_RANGE, "sort: at most 2^27 pairs");
+}
+
+// The sort of (keys, values) in place, with the other ping-pong halves and
+// the histograms in `work` (sort_scratch_bytes(n) bytes). With an odd pass
+// count the result lands in the work arrays and is copied back.
+static void sort_in_place(uint32_t* keys, int32_t* values, int n, int key_bits, void* work, hipStream_t s) {
+    if (n == 0) return;
+    unsigned char* w = static_cast<unsigned char*>(work);
+    uint32_t* kw = reinterpret_cast<uint32_t*>(w);
+    int32_t* vw = reinterpret_cast<int32_t*>(w + align256((size_t)n * 4));
+    uint32_t* hist = reinterpret_cast<uint32_t*>(w + 2 * align256((size_t)n * 4));
+    if (sort_passes(key_bits) % 2 == 0) {
+        launch_radix_sort(keys, kw, values, vw, hist, n, key_bits, s, nullptr, nullptr, 0, values);
+    } else {
+        launch_radix_sort(kw, keys, vw, values, hist, n, key_bits, s, nullptr, nullptr, 0, values);
+        hip_check(hipMemcpyAsync(keys, kw, (size_t)n * 4, hipMemcpyDeviceToDevice, s), "copy");
+        hip_check(hipMemcpyAsync(values, vw, (size_t)n * 4, hipMemcpyDeviceToDevice, s), "copy");
+    }
+    hip_check(hipGetLastError(), "sort launch");
+}
+
+// Entry point bodies (the extern "C" wrappers in sph_abi.cpp convert
+// exceptions to status codes).
+void sort_pairs(uint32_t* keys, int32_t* values, int n, int key_bits, void* stream) {
+    check_args(keys, values, n, key_bits);
+    if (n == 0) return;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    void* work = nullptr;
+    hip_check(hipMallocAsync(&work, sort_scratch_bytes(n), s), "hipMallocAsync");
+    try {
+        sort_in_place(keys, values, n, key_bits, work, s);
+    } catch (...) {
+        (void)hipFreeAsync(work, s);
+        throw;
+    }
+    hip_check(hipFreeAsync(work, s), "hipFreeAsync");
+}
+
+}  // namespace sph

@@ -353,14 +353,6 @@ int sph_set_profiling_filter(sph_handle* h, const char* kernel) {
     });
 }
 
-int sph_set_barrier_timeout(sph_handle* h, double seconds) {
-    return guard([&] {
-        auto* p = dynamic_cast<PCISPHSimulation*>(&sim_of(h));
-        if (!p) throw Error(SPH_ERR_STATE, "barrier timeout applies to PCISPH handles only");
-        p->set_barrier_timeout(seconds);
-        return SPH_OK;
-    });
-}
 
 int sph_get_kernel_times(sph_handle* h, char* names, int* launches, double* total_ms, int cap) {
     return guard([&] {
@@ -949,3 +941,11 @@ int sph_perf_close(sph_perf_log* p) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ sort
+extern "C" int sph_sort_pairs(uint32_t* keys, int32_t* values, int n, int key_bits, void* stream) {
+    return guard([&] {
+        sort_pairs(keys, values, n, key_bits, stream);
+        return SPH_OK;
+    });
+}

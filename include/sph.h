@@ -22,6 +22,8 @@
 #ifndef SPH_H_
 #define SPH_H_
 
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -251,16 +253,17 @@ int sph_set_profiling_filter(sph_handle* h, const char* kernel);
 int sph_get_kernel_times(sph_handle* h, char* names, int* launches, double* total_ms, int cap);
 int sph_reset_kernel_times(sph_handle* h);
 
-/* ----------------------------------------------------------- diagnostics */
+/* ------------------------------------------------------------------ sort */
 
-/* Bound, in seconds, of each grid-barrier wait in a PCISPH step's correction
- * loop (default 5). A wait that reaches it ends the loop's launch and makes
- * the next sph_get_step_stats fail with SPH_ERR_HIP. A value <= 0 gives up at
- * once; it exists so that tests can force that failure from the host.
- * SPH_ERR_STATE for a WCSPH handle. No counterpart in the reference, whose
- * correction loop synchronises with the host after every correction
- * (pcisphsimluation.cpp:129-148). */
-int sph_set_barrier_timeout(sph_handle* h, double seconds);
+/* clsort<cl_uint, cl_int> (opencl/algorithms/clsort.h:12-35): stable sort of
+ * n (key, value) pairs in place by the low key_bits bits of the keys (0 = all
+ * 32, as clogs::Radixsort with maxBits 0), on device arrays the caller owns.
+ * keys (uint32_t) and values (int32_t) are device pointers of n elements;
+ * stream is a hipStream_t (NULL = the null stream). Asynchronous on that
+ * stream: the library takes its work arrays (8 bytes per pair plus the digit
+ * histograms) from the stream-ordered allocator and returns them on the same
+ * stream. n up to 2^27. */
+int sph_sort_pairs(uint32_t* keys, int32_t* values, int n, int key_bits, void* stream);
 
 /* -------------------------------------------------------------- samplers */
 /* Each returns the sample count; out (xyzw) receives up to cap samples. */
