@@ -32,6 +32,20 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+# AddressSanitizer + UndefinedBehaviorSanitizer build (tests/test_sanitizers.py)
+SANITIZE = ["-fsanitize=address,undefined", "-fno-omit-frame-pointer", "-g"]
+
+
+def build_sanitized(out_dir: str) -> str:
+    """The oracle with ASan and UBSan at -O1 into out_dir (g++'s runtimes:
+    load it with libasan preloaded)."""
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "liboracle_asan.so")
+    flags = [f for f in FLAGS if f != "-O3"] + ["-O1"] + SANITIZE
+    subprocess.run(["g++", *flags, os.path.join(HERE, "sph_oracle.cpp"), "-o", out], check=True)
+    return out
+
+
 if __name__ == "__main__":
     build(verbose=True)
     sys.exit(0)

@@ -80,3 +80,45 @@ def test_autotune_cache_round_trip(tmp_path):
     b.step(5)
     for k in range(len(scs)):
         assert np.array_equal(a.positions(k), b.positions(k)), k
+
+
+def test_c4_full_batch_against_oracle_and_single_scenes():
+    """The configured batch: 512 seeded scenes of 1k-64k particles (log-uniform,
+    fluids_amd.scenes.c4_batched), 20 graph-replayed steps. Eight scenes drawn
+    from a seed are stepped by the oracle from the same initial state and must
+    be equal bit for bit (positions, velocities, half-step velocities, and the
+    last step's densities and neighbour counts); all 512 must equal the same
+    scenes run as separate single-scene solvers."""
+    import os
+
+    import oracle
+    from helpers import build_oracle
+    oracle.set_threads(min(16, os.cpu_count() or 1))
+    scs = scenes.c4_batched()
+    assert len(scs) == 512
+    sizes = np.array([sc.n_fluid for sc in scs])
+    print("case c4 full batch", len(scs), int(sizes.sum()), int(sizes.min()), int(sizes.max()), flush=True)
+    b = scenes.build_batch(scs, autotune=False)
+    steps = 20
+    b.step(steps)
+    b.synchronize()
+    pick = np.random.default_rng(404).choice(len(scs), 8, replace=False)
+    pick[0] = int(np.argmax(sizes))  # the largest scene is always among them
+    for k in pick:
+        orc = build_oracle(scs[k])
+        orc.step(steps)
+        label = f"scene {k} (n={scs[k].n_fluid})"
+        assert np.array_equal(b.positions(k), orc.field("pos")), label
+        assert np.array_equal(b.velocities(k), orc.field("vel")), label
+        assert np.array_equal(b.half_velocities(k), orc.field("vel_half")), label
+        assert np.array_equal(b.density(k), orc.field("density")), label
+        assert np.array_equal(b.neighbour_counts(k), orc.field("ncount")), label
+        print("oracle equal:", label, flush=True)
+    for k, sc in enumerate(scs):
+        s = scenes.build(sc)
+        s.step(steps)
+        assert np.array_equal(b.positions(k), s.positions()), k
+        assert np.array_equal(b.velocities(k), s.velocities()), k
+        assert np.array_equal(b.half_velocities(k), s.half_velocities()), k
+        s.close()
+    b.close()

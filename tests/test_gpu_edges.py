@@ -6,7 +6,12 @@
   edge-read counters;
 * ragged particle counts around the wave (64) and workgroup (256) sizes,
   down to a single particle;
-* a step with no fluid particles at all (only the tank).
+* a step with no fluid particles at all (only the tank);
+* the correction loop's host chunks (a relaxation step of 10000
+  corrections);
+* failure detection forced from the host: a NaN velocity (counted NaN
+  positions) and particles set outside the container (the range box grows;
+  edge reads counted as the oracle counts them).
 """
 import numpy as np
 import pytest
@@ -27,6 +32,7 @@ def _step_and_compare(sc, method, steps, fields, label):
         pos, vel = sim.positions(), sim.velocities()
         vh = sim.half_velocities() if method == 0 else None
         orc.set_state(pos, vel, vh)
+        before = orc.edge_reads
         sim.step(1)
         orc.step(1)
         for f in INT_FIELDS:
@@ -40,7 +46,7 @@ def _step_and_compare(sc, method, steps, fields, label):
             assert nbad == 0, f"{label} step {step}: {f} has {nbad} differing words"
         st = sim.step_stats()
         assert st["fluid_list_overflows"] == int(np.sum(orc.field("ncount") >= 50)), label
-        assert st["edge_reads"] == orc.edge_reads, label
+        assert st["edge_reads"] == orc.edge_reads - before, label  # the oracle counts over its lifetime
     return sim, orc
 
 
@@ -88,25 +94,91 @@ def test_no_fluid_particles():
     assert sim.step_stats()["nan_positions"] == 0
 
 
-def test_correction_barrier_timeout_is_reported():
-    """Failure handling of the grid barriers in the PCISPH correction loop,
-    forced from the host: with a zero wait bound every workgroup but the
-    last to arrive gives up at its first barrier, the launch ends, and the
-    step statistics raise instead of returning a silently wrong step. The
-    exit test is made to fail (error ratio -1) so that corrections past the
-    third run; 960 particles give the launch four workgroups."""
-    from fluids_amd._lib import SphError
-    base = scenes.small_pcisph()
+def test_relaxation_runs_every_correction_in_host_chunks():
+    """The relaxation step's cap of 10000 corrections (pcisphsimluation.cpp:
+    548-550) with an exit test that never passes (error ratio -1): the
+    corrections past the third are enqueued eight at a time, the host
+    reading the device's exit flag between chunks. All 10000 run, and the
+    relaxed state and the next step (five corrections) equal the oracle's
+    bit for bit."""
+    base = scenes.small_pcisph(n_side=(8, 8, 6))
     settings = dict(base.settings)
     settings.update(pcisph_max_iterations=5, pcisph_error_ratio=-1.0)
-    sc = Scene("barrier_timeout", settings, base.container, base.volumes, base.bodies)
-    print("case barrier timeout", sc.n_fluid, flush=True)
+    sc = Scene("relaxation_chunks", settings, base.container, base.volumes, base.bodies)
+    print("case relaxation chunks", sc.n_fluid, flush=True)
     sim = scenes.build(sc)
-    assert sim.step_stats()["iterations"] == 10000  # the relaxation step ran every correction
+    orc = build_oracle(sc)
+    assert sim.step_stats()["iterations"] == 10000
+    assert np.array_equal(sim.positions(), orc.field("pos"))
     sim.step(1)
-    assert sim.step_stats()["iterations"] == 5
-    sim.set_barrier_timeout(0.0)
+    orc.step(1)
+    assert sim.step_stats()["iterations"] == orc.iterations == 5
+    assert np.array_equal(sim.positions(), orc.field("pos"))
+    assert np.array_equal(sim.velocities(), orc.field("vel"))
+
+
+def _bits_equal(a, b):
+    """Bit equality of fp32 arrays, any NaN equal to any NaN."""
+    a, b = np.asarray(a, np.float32), np.asarray(b, np.float32)
+    both_nan = np.isnan(a) & np.isnan(b)
+    return bool(np.all((a.view(np.uint32) == b.view(np.uint32)) | both_nan))
+
+
+@pytest.mark.parametrize("method", [1, 0])
+def test_nan_velocity_is_counted(method):
+    """Failure detection forced from the host: one particle's velocity is set
+    to NaN. Within the step every position is still finite (the hash, sort and
+    lists see finite inputs); the NaN spreads through viscosity to the
+    particle's neighbours and into their new positions, which the step
+    statistics count (nan_positions) exactly as many as the oracle produces.
+    Every output, NaN or not, equals the oracle's."""
+    sc = scenes.small_pcisph() if method == 1 else scenes.small_wcsph()
+    print("case nan velocity", method, sc.n_fluid, flush=True)
+    sim = scenes.build(sc)
+    orc = build_oracle(sc)
+    pos, vel = sim.positions(), sim.velocities()
+    vel[17, :3] = np.nan
+    vh = sim.half_velocities() if method == 0 else None
+    sim.set_velocities(vel)
+    orc.set_state(pos, vel, vh)
     sim.step(1)
-    sim.synchronize()
-    with pytest.raises(SphError, match="grid barrier"):
-        sim.step_stats()
+    orc.step(1)
+    for f in INT_FIELDS:
+        assert np.array_equal(sim.field(f), orc.field(f)), f
+    for f in ["pos", "vel", "density", "pressure"]:
+        assert _bits_equal(sim.field(f), orc.field(f)), f
+    want = int(np.sum(~np.isfinite(orc.field("pos")[:, :3]).all(1)))
+    assert want > 1
+    assert sim.step_stats()["nan_positions"] == want
+    # the state now holds NaN positions: this handle is not stepped again
+
+
+def test_positions_outside_the_tank_grow_the_range_box():
+    """Particles set far outside the container (but inside the 5 m periodic
+    grid, one in the last cell column x = cps - 1, whose +x neighbour cells
+    name other cells through the linear CELL_ID or lie past cps^3): the range
+    box grows to hold them, nothing is out of the grid, and the step equals
+    the oracle bit for bit, edge reads included."""
+    sc = scenes.small_pcisph()
+    print("case outside tank", sc.n_fluid, flush=True)
+    sim = scenes.build(sc)
+    orc = build_oracle(sc)
+    pos, vel = sim.positions(), sim.velocities()
+    pos[3, :3] = [2.2, 0.1, 0.0]
+    pos[4, :3] = [-2.3, -0.3, 0.2]
+    pos[5, :3] = [0.1, 0.2, 2.4]
+    pos[6, :3] = [2.495, 2.49, 2.49]
+    sim.set_positions(pos)
+    orc.set_state(pos, vel)
+    before = orc.edge_reads
+    sim.step(1)
+    orc.step(1)
+    for f in INT_FIELDS + ["bkeys", "bperm"]:
+        assert np.array_equal(sim.field(f), orc.field(f)), f
+    for f in ["ranges", "branges"]:
+        assert np.array_equal(sim.field(f), orc.field(f)), f
+    for f in ["density", "pressure", "force_pressure", "pos", "vel"]:
+        assert _bits_equal(sim.field(f), orc.field(f)), f
+    st = sim.step_stats()
+    assert st["out_of_grid"] == 0
+    assert st["edge_reads"] == orc.edge_reads - before > 0

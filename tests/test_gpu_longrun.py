@@ -5,7 +5,8 @@
   density error are checked against the oracle's within 0.5 %.
 * The PCISPH double dam break with the 12 kg cube, two-way coupled to the host
   rigid-body world (sph_coupled_step), at a 0.02 m lattice (87 500 particles)
-  over 300 steps: the oracle runs the same loop with its own world. Both
+  over 300 steps and at its configured 0.01 m lattice (700 000 particles)
+  over 500 steps: the oracle runs the same loop with its own world. Both
   sides add the body wrench in the same pinned order, so the fluid positions
   stay bit-equal and the two worlds receive the same wrenches; the cube's
   trajectory and the fluid's kinetic energy and mean density error are also
@@ -63,9 +64,14 @@ def _world(sc):
     return w, rid
 
 
-def test_pcisph_double_dambreak_coupled_cube():
+@pytest.mark.parametrize("spacing,steps,every", [(0.02, 300, 50), (0.01, 500, 100)])
+def test_pcisph_double_dambreak_coupled_cube(spacing, steps, every):
+    """0.02 m (87 500 particles, 300 steps) and the configured run: 0.01 m
+    lattice (700 000 particles), the 12 kg cube sampled at 0.01 m, 500 steps
+    (data/scenes/double_dambreak.json, data/scenes/many_objects.json:31-37)."""
     _threads()
-    sc = scenes.c2_pcisph_double_dambreak(spacing=0.02)
+    sc = scenes.c2_pcisph_double_dambreak(spacing=spacing)
+    print("case double dam break", spacing, sc.n_fluid, sc.n_boundary, steps, flush=True)
     sim = scenes.build(sc)
     orc = build_oracle(sc)
     wg, rg = _world(sc)
@@ -74,7 +80,7 @@ def test_pcisph_double_dambreak_coupled_cube():
     dt = sc.settings["time_step"]
     start = wg.state(rg)["com"].astype(np.float64)
     max_dev = 0.0
-    for step in range(1, 301):
+    for step in range(1, steps + 1):
         coupled_step(sim, wg, [(0, rg)])
         # the oracle runs Scene::render's order in Python with its own world
         s = wo.state(ro)
@@ -83,7 +89,7 @@ def test_pcisph_double_dambreak_coupled_cube():
         f, t, _, _ = orc.body_wrench(0)
         wo.apply_force_n_torque(ro, f, t)
         wo.step(dt)
-        if step % 50 == 0:
+        if step % every == 0:
             pg, po = sim.positions(), orc.field("pos")
             assert np.array_equal(pg, po), f"step {step}: positions differ in {np.sum(pg != po)} values"
             cg, co = wg.state(rg), wo.state(ro)
