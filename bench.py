@@ -11,13 +11,16 @@ the run as the fluid boxes collapse and spread over the floor.
 Roofline: algorithmic bytes per particle-update follow the unique-bytes model
 of the reference dataflow (each array a reference kernel reads or writes
 counted once per launch, neighbour gathers as cache hits, list indices int32
-at the mean neighbour counts K and Kb measured after the timed steps). The
+at the mean neighbour counts K and Kb over the timed steps). The
 dominant kernel's
 achieved bandwidth is its algorithmic bytes per launch over its mean launch
 time measured with HIP events on the solver's stream inside the timed region.
 
 cpu_baseline: the CPU restatement of the reference kernels (oracle/, fp32,
-same kernel sequence, OpenMP) on the same scene, timed over a few steps.
+same kernel sequence, OpenMP), built with -march=native on the host that runs
+the benchmark, on the same scene, timed over a few steps with every host CPU
+this process may use up to the 16 a job is granted (nproc, the allowed CPUs
+and the CPU model are recorded).
 
 fast_math: the same scene and step count with SPH_FAST_MATH (approximate
 reciprocal and square root in the PCISPH kernels, as the reference compiles
@@ -62,6 +65,41 @@ def step_bytes(K, Kb, n_it, N, B, B_moving):
     """B_pci per particle-update, reference dataflow."""
     b = 429 + 168 * n_it + (16 + 8 * n_it) * K + (8 + 8 * n_it) * Kb
     return b + (108.0 * B + 132.0 * B_moving) / N
+
+
+def cpu_description():
+    """nproc, the CPUs this process may run on, and the host's CPU model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    return {"nproc": os.cpu_count(), "allowed_cpus": allowed, "model": model}
+
+
+def native_oracle():
+    """Builds the oracle with -march=native for this host (a temporary copy
+    beside the shipped x86-64-v3 build) and points the oracle loader at it.
+    Returns the build flags used, or None if the native build failed."""
+    import subprocess
+    import tempfile
+    from oracle import build as ob
+    out = os.path.join(tempfile.mkdtemp(prefix="oracle_native_"), "liboracle_native.so")
+    flags = [f if not f.startswith("-march=") else "-march=native" for f in ob.FLAGS]
+    r = subprocess.run(["g++", *flags, os.path.join(ROOT, "oracle", "sph_oracle.cpp"), "-o", out],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        return None
+    os.environ["SPH_ORACLE_LIB"] = out
+    return " ".join(flags)
 
 
 def run_cpu_baseline(sc, steps, threads):
@@ -129,6 +167,9 @@ def run_batched(steps, warmup, plain_steps, count=512):
     wall = time.perf_counter() - t0
     shapes = {name: b.launch(name) for name in ("batch_hash", "batch_gather_ranges", "batch_lists_density",
                                                 "batch_normals", "batch_accel_integrate")}
+    # mean fluid neighbour count over every particle of the batch after the
+    # timed steps (no boundary particles: Kb = 0)
+    K = sum(float(b.neighbour_counts(k).sum()) for k in range(len(scs))) / total
     b.close()
     sims = [scenes.build(sc) for sc in scs]
     for sm in sims:
@@ -146,8 +187,11 @@ def run_batched(steps, warmup, plain_steps, count=512):
         sm.close()
     v = total * steps / wall
     vp = total * plain_steps / wall_plain
+    b_wc = 473.0 + 16.0 * K  # B_wc per particle-update, Kb = 0
+    roof = {"bound": "hbm", "alg_bytes_per_pu": b_wc, "mean_fluid_neighbours": K,
+            "achieved": b_wc * v / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": b_wc * v / HBM_PEAK}
     return {"workload": f"batched_wcsph_{count}", "scenes": count, "particles": total, "steps": steps,
-            "value": v, "unit": "particle-updates/s", "ms_per_step": 1e3 * wall / steps,
+            "value": v, "unit": "particle-updates/s", "ms_per_step": 1e3 * wall / steps, "roofline": roof,
             "plain_value": vp, "plain_steps": plain_steps, "plain_ms_per_step": 1e3 * wall_plain / plain_steps,
             "speedup_vs_plain": v / vp, "launch_shapes": shapes, "build_and_tune_s": t_tune}
 
@@ -259,6 +303,10 @@ def main():
     # timed region: HIP events bracket only the dominant kernel's launches
     sim.reset_kernel_times()
     sim.set_profiling(not a.no_events and dom is not None, kernel=dom)
+    # the state the timed steps start from, to replay them for their counts
+    pos0, vel0 = sim.positions(), sim.velocities()
+    sim.set_positions(pos0)
+    sim.set_velocities(vel0)
     sim.synchronize()
     t0 = time.perf_counter()
     sim.step(a.steps)
@@ -267,11 +315,25 @@ def main():
     sim.set_profiling(False)
     kt = sim.kernel_times()
     st = sim.step_stats()
+    pos_end = sim.positions()
 
+    # K, Kb and the corrections as the mean over the timed steps: the same
+    # steps replayed (untimed) from the saved state give the same bits, with
+    # the step statistics read after each
     N, B = sc.n_fluid, sc.n_boundary
-    n_it = st["iterations"]
-    K = st["fluid_neighbours"] / N
-    Kb = st["boundary_neighbours"] / N
+    sim.set_positions(pos0)
+    sim.set_velocities(vel0)
+    fl = bl = its = 0
+    for _ in range(a.steps):
+        sim.step(1)
+        s1 = sim.step_stats()
+        fl += s1["fluid_neighbours"]
+        bl += s1["boundary_neighbours"]
+        its += s1["iterations"]
+    replay_identical = bool((sim.positions() == pos_end).all())
+    n_it = its / a.steps
+    K = fl / (N * a.steps)
+    Kb = bl / (N * a.steps)
     pu_s = N * a.steps / wall
     bpp = step_bytes(K, Kb, n_it, N, B, 0)
 
@@ -296,11 +358,16 @@ def main():
 
     cpu = None
     if not a.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
+        host = cpu_description()
+        # a job on the GPU host is granted at most 16 host threads, whatever
+        # nproc shows; every CPU this process may use, up to that share
+        threads = max(1, min(host["allowed_cpus"], 16))
+        flags = native_oracle()
         v, dt = run_cpu_baseline(sc, a.cpu_steps, threads)
         cpu = {"value": v, "unit": "particle-updates/s", "cores": threads, "kind": "port",
+               "host": host, "build": flags or "shipped x86-64-v3 build (native build failed)",
                "sample": f"{a.cpu_steps} PCISPH steps of the same {N}-particle scene after its relaxation step "
-                         f"({dt:.1f} s), fp32 oracle, OpenMP"}
+                         f"({dt:.1f} s), fp32 oracle, OpenMP with {threads} threads"}
 
     del sim
     fast = None if a.no_fast_math else run_fast_math(a.n, a.steps, a.warmup, a.max_iter)
@@ -316,8 +383,10 @@ def main():
         "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"pcisph_throughput_{N // 1000}k", "fluid_particles": N, "boundary_particles": B,
-                   "spacing_m": 0.005, "support_radius_m": 0.01, "corrections": n_it,
-                   "mean_fluid_neighbours": round(K, 3), "mean_boundary_neighbours": round(Kb, 3)},
+                   "spacing_m": 0.005, "support_radius_m": 0.01, "corrections": round(n_it, 3),
+                   "mean_fluid_neighbours": round(K, 3), "mean_boundary_neighbours": round(Kb, 3),
+                   "counts": "means over the timed steps (replayed untimed from the same state; "
+                             f"replay bit-identical: {replay_identical})"},
         "roofline": roof,
         "step_roofline": {"alg_bytes_per_pu": bpp, "achieved_GBps": bpp * pu_s / 1e9, "frac": bpp * pu_s / HBM_PEAK},
         "cpu_baseline": cpu,
