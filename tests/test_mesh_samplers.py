@@ -5,14 +5,16 @@
   generator is pinned to the C++ standard's known answer for minstd_rand0;
   the icosphere's vertices lie on the sphere and number 12 + 3 * (20 + 80 +
   320) = 1272.
-* Model::surface_sampling: 8 corners per voxel; the voxel set equals the one
-  found by clipping each triangle against each candidate box in float64
-  (a different overlap algorithm from the library's separating axes),
-  on the teapot model of the reference's data (a committed fixture).
+* Model::surface_sampling: the samples equal, in count, order and bits, a
+  float32 restatement of the reference's voxelizer (oracle/mesh_samplers.py
+  voxelize, with its own separating-axis test) on the reference's teapot and
+  bunny models (committed fixtures); that overlap test is in turn pinned to
+  float64 clipping of each triangle against each candidate box.
 """
 import os
 
 import numpy as np
+import pytest
 
 from fluids_amd import model_sampling, sphere_sampling
 from oracle import mesh_samplers as ms
@@ -48,25 +50,35 @@ def test_sphere_sampling_matches_restatement():
         assert np.abs(tail - radius).max() < 1e-6 * max(radius, 1)
 
 
-def test_model_sampling_voxel_set():
-    path = os.path.join(GOLDEN, "teapot.obj")
-    s = 0.02
-    a = model_sampling(path, s)
-    assert a.shape[0] % 8 == 0 and a.shape[0] > 0
+@pytest.mark.parametrize("model,spacing", [("teapot.obj", 0.02), ("teapot.obj", 0.05), ("bunny.obj", 0.01),
+                                           ("bunny.obj", 0.02)])
+def test_model_sampling_exact(model, spacing):
+    """Model::surface_sampling (scene/model.cpp:170-214 over vx_voxelize,
+    voxelizer.h:629-760) on two of the reference's models: the library's
+    samples equal the float32 restatement's (oracle/mesh_samplers.py
+    voxelize) in count, order and bits; 8 corners per voxel, each the
+    voxel centre +- s/2."""
+    path = os.path.join(GOLDEN, model)
+    a = model_sampling(path, spacing)
+    verts, tris = ms.read_obj(path)
+    b = ms.voxelize(verts, tris, spacing, spacing / 10)
+    print(f"{model} at {spacing}: {a.shape[0]} samples ({a.shape[0] // 8} voxels)")
+    assert a.shape == b.shape and a.shape[0] > 0 and a.shape[0] % 8 == 0
+    assert np.array_equal(a, b)
     corners = a[:, :3].reshape(-1, 8, 3).astype(np.float64)
     centres = corners.mean(axis=1)
-    # each voxel's 8 corners are the centre +- s/2
-    assert np.abs(np.abs(corners - centres[:, None, :]) - s / 2).max() < 1e-6
-    # voxel centres come from float loops started at each triangle's snapped
-    # corner, so one lattice voxel can be met at centres a few ulp apart and
-    # (beyond the 1e-7 tolerance of the voxelizer's table) kept twice, as in
-    # the reference
-    idx = np.rint(centres / s).astype(int)
-    assert np.abs(centres - idx * s).max() < 1e-5
-    mine = {tuple(i) for i in idx}
-    verts, tris = ms.read_obj(path)
-    ref = ms.voxel_centres(verts, tris, s, s / 10)
-    diff = mine ^ ref
-    print(f"voxels: library {len(mine)} ({centres.shape[0]} with repeats), clipping {len(ref)}, "
-          f"differing {len(diff)}")
-    assert len(diff) <= max(2, len(ref) // 500)
+    assert np.abs(np.abs(corners - centres[:, None, :]) - spacing / 2).max() < 1e-6
+
+
+def test_voxel_overlap_test_against_clipping():
+    """The restatement's separating-axis test (the one the samples above are
+    pinned to) against float64 clipping of each triangle to each candidate
+    box of the teapot at 0.02 m: every box that the triangle meets when
+    shrunk by 1e-6 is kept, every box it misses when grown by 1e-6 is not,
+    with no disagreement (boxes within 1e-6 of touching are counted apart)."""
+    verts, tris = ms.read_obj(os.path.join(GOLDEN, "teapot.obj"))
+    agree_in, agree_out, disagree, ambiguous = ms.sat_against_clipping(verts, tris, 0.02, 0.002)
+    print(f"overlap {agree_in}, separated {agree_out}, disagreeing {disagree}, within rounding {ambiguous}")
+    assert disagree == 0
+    assert agree_in > 5000 and agree_out > 5000
+    assert ambiguous <= (agree_in + agree_out) // 1000
