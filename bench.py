@@ -248,6 +248,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-events", action="store_true", help="time without per-kernel events")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--json-out", default=None)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-scenes measurement")
     ap.add_argument("--batched-only", action="store_true", help="only the batched-scenes measurement")
@@ -285,6 +286,13 @@ def main():
     sim.synchronize()
     t_init = time.perf_counter() - t_init
     print(f"# scene {sc.name}: N={sc.n_fluid} B={sc.n_boundary} init {t_init:.1f}s", file=sys.stderr, flush=True)
+
+    shapes = None
+    if not a.no_autotune:
+        # launch shapes of the list kernels, timed on this scene (restores the state)
+        sim.autotune(None, trials=3)
+        shapes = {k: sim.launch(k) for k in sim.TUNED_KERNELS}
+        print(f"# tuned launch shapes: {shapes}", file=sys.stderr, flush=True)
 
     sim.step(a.warmup)
     sim.synchronize()
@@ -394,6 +402,7 @@ def main():
         "batched": batched,
         "adjoint": adjoint,
         "kernels": per,
+        "launch_shapes": shapes,
         "events_in_timed_region": not a.no_events,
     }
     line = json.dumps(rec)

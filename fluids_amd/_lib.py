@@ -86,6 +86,9 @@ SIGNATURES = {
     "sph_get_step_stats": (I, [P, C.POINTER(SphStepStats)]),
     "sph_device_state": (I, [P, P, P, P]),
     "sph_stream": (P, [P]),
+    "sph_autotune": (I, [P, C.c_char_p, I]),
+    "sph_get_launch": (I, [P, I, P, P]),
+    "sph_set_launch": (I, [P, I, I, I]),
     "sph_sort_pairs": (I, [P, P, I, I, P]),
     "sph_set_profiling": (I, [P, I]),
     "sph_set_profiling_filter": (I, [P, C.c_char_p]),

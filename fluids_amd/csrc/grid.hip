@@ -839,11 +839,11 @@ __global__ __launch_bounds__(256) void k_normals(int n, const float4* __restrict
 }
 
 void launch_normals(int n, const float4* prho, const int* nl, const int* nc, float H2, float w_grad, float M,
-                    float H, float4* normal, hipStream_t s, Profiler* prof, bool fast) {
+                    float H, float4* normal, hipStream_t s, Profiler* prof, bool fast, Shape sh) {
     if (n <= 0) return;
     PROF_BEGIN(prof, "normals", s);
-    hipLaunchKernelGGL(fast ? k_normals<true> : k_normals<false>, dim3((n + 255) / 256), dim3(256), 0, s, n, prho, nl,
-                       nc, H2, w_grad, M, H, normal);
+    hipLaunchKernelGGL(fast ? k_normals<true> : k_normals<false>, dim3((n + sh.block - 1) / sh.block),
+                       dim3(sh.block), 0, s, n, prho, nl, nc, H2, w_grad, M, H, normal);
     PROF_END(prof, s);
 }
 

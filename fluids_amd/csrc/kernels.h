@@ -83,6 +83,14 @@ struct WcParams {
     int fast;  // 1: Math<true> build of the single-scene kernels (SPH_FAST_MATH); batches use the IEEE build
 };
 
+// Launch shape of a single-scene kernel: threads per block (64, 128 or 256)
+// and, for the grid-stride kernels, blocks (0: one thread per particle).
+// Shapes change timing only, never results.
+struct Shape {
+    int block = 256;
+    int grid = 0;
+};
+
 // grid.hip
 int sort_passes(int key_bits);
 // passes of a sort whose last pass orders by scene id (sid_bits > 0)
@@ -116,7 +124,7 @@ void launch_lists_density(bool wcsph, int n, const float4* pos_s, const int2* fR
                           float* pressure, float4* prho, DevState* st, hipStream_t s, Profiler* prof);
 // fast: the Math<true> build (SPH_FAST_MATH)
 void launch_normals(int n, const float4* prho, const int* nl, const int* nc, float H2, float w_grad, float M,
-                    float H, float4* normal, hipStream_t s, Profiler* prof, bool fast = false);
+                    float H, float4* normal, hipStream_t s, Profiler* prof, bool fast = false, Shape sh = Shape{});
 void launch_step_begin(DevState* st, hipStream_t s);
 void launch_fill_u32(void* p, size_t n, uint32_t v, hipStream_t s);
 void launch_stats(int n, const int* nc, const int* bnc, const float4* pos, unsigned long long* acc, hipStream_t s);
@@ -127,15 +135,16 @@ void sort_pairs(uint32_t* keys, int32_t* values, int n, int key_bits, void* stre
 // pcisph.hip
 void launch_forces_predict(int n, const float4* prho, const float4* vel_s, const float4* normal, const int* nl,
                            const int* nc, const PciParams& P, float4* f_other, float4* pos_pred, hipStream_t s,
-                           Profiler* prof);
+                           Profiler* prof, Shape sh = Shape{});
 void launch_update_pressure(int n, int iter, const float4* pos_pred, const float4* prho, const int* nl,
                             const int* nc, const int* bnl, const int* bnc, const float4* bpos_s, const float* bphi_s,
                             int has_b, const PciParams& P, float* err, float* pressure, float4* pc, DevState* st,
-                            hipStream_t s, Profiler* prof);
+                            hipStream_t s, Profiler* prof, Shape sh = Shape{});
 void launch_pressure_force(int n, int iter, const float4* pc, const float4* vel_s, const float4* f_other,
                            const int* nl, const int* nc, const int* bnl, const int* bnc, const float4* bpos_s,
                            const float* bphi_s, int has_b, const PciParams& P, float4* f_press, float4* pos_pred,
-                           float4* pos_out, float4* vel_out, DevState* st, hipStream_t s, Profiler* prof);
+                           float4* pos_out, float4* vel_out, DevState* st, hipStream_t s, Profiler* prof,
+                           Shape sh = Shape{});
 // Corrections first .. last, one launch per phase on a grid-stride grid of at
 // most 256 workgroups; each returns at once when the loop already ended.
 void launch_corrections(int n, int first, int last, const float4* prho, const float4* vel_s, const float4* f_other,

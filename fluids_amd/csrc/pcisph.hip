@@ -357,6 +357,12 @@ __global__ __launch_bounds__(256) void k_integrate_only(int n, const float4* __r
 // Corrections up to min_iter always run and get one thread per particle;
 // later ones run in k_correction_tail.
 static int correction_blocks(int n) { return (n + 255) / 256; }
+// Blocks of a grid-stride correction launch: a fixed grid, or one thread per
+// particle.
+static int shape_blocks(int n, Shape sh) {
+    const int full = (n + sh.block - 1) / sh.block;
+    return sh.grid > 0 && sh.grid < full ? sh.grid : full;
+}
 
 // Grid of the launches for corrections past the ones that always run: they
 // usually return at once (the exit test ended the loop), so a small
@@ -365,22 +371,22 @@ constexpr int TAIL_BLOCKS = 256;
 
 void launch_forces_predict(int n, const float4* prho, const float4* vel_s, const float4* normal, const int* nl,
                            const int* nc, const PciParams& P, float4* f_other, float4* pos_pred, hipStream_t s,
-                           Profiler* prof) {
+                           Profiler* prof, Shape sh) {
     if (n <= 0) return;
     PROF_BEGIN(prof, "forces_predict", s);
-    hipLaunchKernelGGL(P.fast ? k_forces_predict<true> : k_forces_predict<false>, dim3((n + 255) / 256), dim3(256), 0,
-                       s, n, prho, vel_s, normal, nl, nc, P, f_other, pos_pred);
+    hipLaunchKernelGGL(P.fast ? k_forces_predict<true> : k_forces_predict<false>, dim3((n + sh.block - 1) / sh.block),
+                       dim3(sh.block), 0, s, n, prho, vel_s, normal, nl, nc, P, f_other, pos_pred);
     PROF_END(prof, s);
 }
 
 void launch_update_pressure(int n, int iter, const float4* pos_pred, const float4* prho, const int* nl,
                             const int* nc, const int* bnl, const int* bnc, const float4* bpos_s, const float* bphi_s,
                             int has_b, const PciParams& P, float* err, float* pressure, float4* pc, DevState* st,
-                            hipStream_t s, Profiler* prof) {
+                            hipStream_t s, Profiler* prof, Shape sh) {
     if (n <= 0) return;
     PROF_BEGIN(prof, "update_pressure", s);
-    hipLaunchKernelGGL(P.fast ? k_update_pressure<true> : k_update_pressure<false>, dim3(correction_blocks(n)),
-                       dim3(256), 0, s, n, iter, pos_pred, prho, nl, nc, bnl, bnc, bpos_s, bphi_s, has_b, P, err,
+    hipLaunchKernelGGL(P.fast ? k_update_pressure<true> : k_update_pressure<false>, dim3(shape_blocks(n, sh)),
+                       dim3(sh.block), 0, s, n, iter, pos_pred, prho, nl, nc, bnl, bnc, bpos_s, bphi_s, has_b, P, err,
                        pressure, pc, st);
     PROF_END(prof, s);
 }
@@ -388,11 +394,12 @@ void launch_update_pressure(int n, int iter, const float4* pos_pred, const float
 void launch_pressure_force(int n, int iter, const float4* pc, const float4* vel_s, const float4* f_other,
                            const int* nl, const int* nc, const int* bnl, const int* bnc, const float4* bpos_s,
                            const float* bphi_s, int has_b, const PciParams& P, float4* f_press, float4* pos_pred,
-                           float4* pos_out, float4* vel_out, DevState* st, hipStream_t s, Profiler* prof) {
+                           float4* pos_out, float4* vel_out, DevState* st, hipStream_t s, Profiler* prof,
+                           Shape sh) {
     if (n <= 0) return;
     PROF_BEGIN(prof, "pressure_force", s);
-    hipLaunchKernelGGL(P.fast ? k_pressure_force<true> : k_pressure_force<false>, dim3(correction_blocks(n)),
-                       dim3(256), 0, s, n, iter, pc, vel_s, f_other, nl, nc, bnl, bnc, bpos_s, bphi_s, has_b, P,
+    hipLaunchKernelGGL(P.fast ? k_pressure_force<true> : k_pressure_force<false>, dim3(shape_blocks(n, sh)),
+                       dim3(sh.block), 0, s, n, iter, pc, vel_s, f_other, nl, nc, bnl, bnc, bpos_s, bphi_s, has_b, P,
                        f_press, pos_pred, pos_out, vel_out, st);
     PROF_END(prof, s);
 }

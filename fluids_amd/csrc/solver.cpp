@@ -861,19 +861,19 @@ void PCISPHSimulation::simulate_step(int min_iter, int max_iter) {
                          d_nl.get(), d_nc.get(), d_bnl.get(), d_bnc.get(), d_density.get(), nullptr, d_prho.get(),
                          d_state.get(), s, &prof_);
     launch_normals(n_, d_prho.get(), d_nl.get(), d_nc.get(), P_.H2, P_.w_grad, P_.M, P_.H, d_normal.get(), s,
-                   &prof_, P_.fast != 0);
+                   &prof_, P_.fast != 0, shapes_[T_NORMALS]);
     if (max_iter <= 0) {
         launch_fill_u32(d_pressure.get(), n_, 0u, s);
         launch_fill_u32(d_f_press.get(), (size_t)n_ * 4, 0u, s);
         launch_forces_predict(n_, d_prho.get(), d_vel_s.get(), d_normal.get(), d_nl.get(), d_nc.get(), P_,
-                              d_f_other.get(), d_pos_pred.get(), s, &prof_);
+                              d_f_other.get(), d_pos_pred.get(), s, &prof_, shapes_[T_FORCES]);
         boundary_.apply_fluid_forces();
         launch_pci_integrate_only(n_, d_pos_s.get(), d_vel_s.get(), d_f_other.get(), P_, d_pos.get(), d_vel.get(), s,
                                   &prof_);
         return;
     }
     launch_forces_predict(n_, d_prho.get(), d_vel_s.get(), d_normal.get(), d_nl.get(), d_nc.get(), P_,
-                          d_f_other.get(), d_pos_pred.get(), s, &prof_);
+                          d_f_other.get(), d_pos_pred.get(), s, &prof_, shapes_[T_FORCES]);
     // The corrections that always run get one launch per phase on full grids;
     // the rest of the loop, up to max_iter, on small grids that return at once
     // after the device-side exit test ended the loop, so a step needs no host
@@ -884,10 +884,11 @@ void PCISPHSimulation::simulate_step(int min_iter, int max_iter) {
     for (int i = 1; i <= always; ++i) {
         launch_update_pressure(n_, i, d_pos_pred.get(), d_prho.get(), d_nl.get(), d_nc.get(), d_bnl.get(),
                                d_bnc.get(), boundary_.d_pos_s.get(), boundary_.d_phi_s.get(), has_b, P_, d_err.get(),
-                               d_pressure.get(), d_pc.get(), d_state.get(), s, &prof_);
+                               d_pressure.get(), d_pc.get(), d_state.get(), s, &prof_, shapes_[T_UPDATE]);
         launch_pressure_force(n_, i, d_pc.get(), d_vel_s.get(), d_f_other.get(), d_nl.get(), d_nc.get(), d_bnl.get(),
                               d_bnc.get(), boundary_.d_pos_s.get(), boundary_.d_phi_s.get(), has_b, P_,
-                              d_f_press.get(), d_pos_pred.get(), d_pos.get(), d_vel.get(), d_state.get(), s, &prof_);
+                              d_f_press.get(), d_pos_pred.get(), d_pos.get(), d_vel.get(), d_state.get(), s, &prof_,
+                              shapes_[T_PFORCE]);
     }
     constexpr int TAIL_CHUNK = 8;
     for (int first = always + 1; first <= max_iter; first += TAIL_CHUNK) {
@@ -905,6 +906,119 @@ void PCISPHSimulation::simulate_step(int min_iter, int max_iter) {
                            d_vel.get(), d_state.get(), s, &prof_);
     }
     boundary_.apply_fluid_forces();
+}
+
+// ------------------------------------------------------------ autotuning
+const char* PCISPHSimulation::tuned_name(int k) {
+    static const char* names[T_COUNT] = {"normals", "forces_predict", "update_pressure", "pressure_force"};
+    return (k >= 0 && k < T_COUNT) ? names[k] : "?";
+}
+
+void PCISPHSimulation::set_shape(int k, Shape sh) {
+    if (k < 0 || k >= T_COUNT) throw Error(SPH_ERR_RANGE, "no such tuned kernel");
+    if (sh.block != 64 && sh.block != 128 && sh.block != 256)
+        throw Error(SPH_ERR_INVALID, "threads per block must be 64, 128 or 256");
+    if (sh.grid < 0 || ((k == T_NORMALS || k == T_FORCES) && sh.grid != 0))
+        throw Error(SPH_ERR_INVALID, "a fixed grid applies to update_pressure and pressure_force only");
+    shapes_[k] = sh;
+}
+
+std::string PCISPHSimulation::tune_key() const {
+    hipDeviceProp_t prop{};
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    hip_check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    std::string arch = prop.gcnArchName;
+    for (char& c : arch)
+        if (c == ' ') c = '_';
+    return "pcisph " + arch + " cu" + std::to_string(prop.multiProcessorCount) + " n" + std::to_string(n_) + " b" +
+           std::to_string(boundary_.count());
+}
+
+int PCISPHSimulation::autotune(const char* cache_path, int trials) {
+    require_ready();
+    if (n_ == 0) return 0;
+    const std::string key = tune_key();
+    if (cache_path && *cache_path) {
+        std::ifstream f(cache_path);
+        std::string line;
+        Shape found[T_COUNT];
+        int seen = 0;
+        while (std::getline(f, line)) {
+            if (line.compare(0, key.size() + 1, key + " ") != 0) continue;
+            std::istringstream ss(line.substr(key.size() + 1));
+            std::string name;
+            Shape sh;
+            if (!(ss >> name >> sh.block >> sh.grid)) continue;
+            for (int k = 0; k < T_COUNT; ++k)
+                if (name == tuned_name(k)) {
+                    found[k] = sh;
+                    seen |= 1 << k;
+                }
+        }
+        if (seen == (1 << T_COUNT) - 1) {
+            for (int k = 0; k < T_COUNT; ++k) set_shape(k, found[k]);
+            return 1;
+        }
+    }
+    if (trials < 1) trials = 1;
+    hipDeviceProp_t prop{};
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    hip_check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    const int cus = std::max(prop.multiProcessorCount, 1);
+    // the tuning steps advance the fluid; the caller's steps continue from
+    // the state before them (a step depends on positions and velocities only)
+    synchronize();
+    std::vector<float4> save_p(n_), save_v(n_);
+    get_state(0, &save_p[0].x, n_);
+    get_state(1, &save_v[0].x, n_);
+    const bool prof_was = prof_.enabled;
+    const std::string filter_was = prof_.filter;
+    const int blocks[3] = {64, 128, 256};
+    for (int k = 0; k < T_COUNT; ++k) {
+        std::vector<Shape> cands;
+        for (int b : blocks) {
+            cands.push_back(Shape{b, 0});
+            if (k == T_UPDATE || k == T_PFORCE) {
+                cands.push_back(Shape{b, 4 * cus});
+                cands.push_back(Shape{b, 16 * cus});
+            }
+        }
+        Shape best = shapes_[k];
+        double best_ms = 1e30;
+        for (const Shape& c : cands) {
+            shapes_[k] = c;
+            prof_.enabled = false;
+            simulate_step(3, s_.pcisph_max_iterations);  // warm
+            prof_.reset();
+            prof_.filter = tuned_name(k);
+            prof_.enabled = true;
+            for (int t = 0; t < trials; ++t) simulate_step(3, s_.pcisph_max_iterations);
+            synchronize();
+            prof_.resolve();
+            prof_.enabled = false;
+            for (auto& e : prof_.entries)
+                if (e.name == tuned_name(k) && e.launches > 0 && e.total_ms / e.launches < best_ms) {
+                    best_ms = e.total_ms / e.launches;
+                    best = c;
+                }
+        }
+        shapes_[k] = best;
+    }
+    prof_.reset();
+    prof_.filter = filter_was;
+    prof_.enabled = prof_was;
+    set_state(0, &save_p[0].x, n_);
+    set_state(1, &save_v[0].x, n_);
+    synchronize();
+    if (cache_path && *cache_path) {
+        std::ofstream f(cache_path, std::ios::app);
+        if (!f.good()) throw Error(SPH_ERR_INVALID, "autotune: cannot write the cache " + std::string(cache_path));
+        for (int k = 0; k < T_COUNT; ++k)
+            f << key << " " << tuned_name(k) << " " << shapes_[k].block << " " << shapes_[k].grid << "\n";
+    }
+    return 0;
 }
 
 // ------------------------------------------------------------------ WCSPH

@@ -262,10 +262,25 @@ public:
     float density_scale_factor() const { return delta_; }
     PciParams params() const { return P_; }
 
+    // Launch shapes of the step's list kernels (the single-scene form of the
+    // reference's launch autotuning, opencl/clkernel.cpp:85-151): shapes
+    // change timing only, never results.
+    enum TunedKernel { T_NORMALS, T_FORCES, T_UPDATE, T_PFORCE, T_COUNT };
+    static const char* tuned_name(int k);
+    Shape shape(int k) const { return shapes_[k]; }
+    void set_shape(int k, Shape sh);
+    // Times each kernel's candidate shapes over `trials` steps from the
+    // current state, keeps the fastest, restores the state; with cache_path,
+    // earlier results for the same device and scene size are reused (returns
+    // 1) or the new ones appended (returns 0).
+    int autotune(const char* cache_path, int trials);
+
 protected:
     void initialize_params() override;
     void after_initialize() override;
     void simulate_step(int min_iter, int max_iter);
+    std::string tune_key() const;
+    Shape shapes_[T_COUNT];
     void deduce_density_scale_factor();
     void fill_params();
     float mass_ = 0.0f, st_main_ = 0.0f, st_term_ = 0.0f, delta_ = 0.0f;

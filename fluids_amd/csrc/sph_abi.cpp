@@ -942,6 +942,38 @@ int sph_perf_close(sph_perf_log* p) {
 
 }  // extern "C"
 
+// ------------------------------------------------------------ autotuning
+namespace {
+PCISPHSimulation* pcisph_of(sph_handle* h) {
+    if (!h) throw Error(SPH_ERR_INVALID, "null handle");
+    auto* p = dynamic_cast<PCISPHSimulation*>(h->sim.get());
+    if (!p) throw Error(SPH_ERR_STATE, "launch shapes are tuned for PCISPH handles");
+    return p;
+}
+}  // namespace
+
+extern "C" int sph_autotune(sph_handle* h, const char* cache_path, int trials) {
+    return guard([&] { return pcisph_of(h)->autotune(cache_path, trials); });
+}
+
+extern "C" int sph_get_launch(sph_handle* h, int kernel, int* block, int* grid) {
+    return guard([&] {
+        auto* p = pcisph_of(h);
+        if (kernel < 0 || kernel >= PCISPHSimulation::T_COUNT) throw Error(SPH_ERR_RANGE, "no such tuned kernel");
+        Shape sh = p->shape(kernel);
+        if (block) *block = sh.block;
+        if (grid) *grid = sh.grid;
+        return SPH_OK;
+    });
+}
+
+extern "C" int sph_set_launch(sph_handle* h, int kernel, int block, int grid) {
+    return guard([&] {
+        pcisph_of(h)->set_shape(kernel, Shape{block, grid});
+        return SPH_OK;
+    });
+}
+
 // ------------------------------------------------------------------ sort
 extern "C" int sph_sort_pairs(uint32_t* keys, int32_t* values, int n, int key_bits, void* stream) {
     return guard([&] {

@@ -224,6 +224,24 @@ class FluidSimulation:
         return {k: getattr(s, k) for k, _ in SphStepStats._fields_}
 
 
+    TUNED_KERNELS = ("normals", "forces_predict", "update_pressure", "pressure_force")
+
+    def autotune(self, cache: str = None, trials: int = 3) -> bool:
+        """Times the PCISPH list kernels' launch shapes and keeps the fastest
+        (sph_autotune); True if the cache supplied them."""
+        return check(lib().sph_autotune(self._h, None if cache is None else cache.encode(), int(trials))) == 1
+
+    def launch(self, kernel) -> tuple:
+        """(threads per block, grid) of a tuned kernel (name or SPH_TUNE_* id)."""
+        k = self.TUNED_KERNELS.index(kernel) if isinstance(kernel, str) else int(kernel)
+        b, g = C.c_int(), C.c_int()
+        check(lib().sph_get_launch(self._h, k, C.byref(b), C.byref(g)))
+        return b.value, g.value
+
+    def set_launch(self, kernel, block: int, grid: int = 0):
+        k = self.TUNED_KERNELS.index(kernel) if isinstance(kernel, str) else int(kernel)
+        check(lib().sph_set_launch(self._h, k, int(block), int(grid)))
+
     def set_profiling(self, on=True, kernel: str = None):
         """HIP-event timing of every launch, or of one kernel's launches only."""
         check(lib().sph_set_profiling_filter(self._h, (kernel or "").encode()))

@@ -253,6 +253,29 @@ int sph_set_profiling_filter(sph_handle* h, const char* kernel);
 int sph_get_kernel_times(sph_handle* h, char* names, int* launches, double* total_ms, int cap);
 int sph_reset_kernel_times(sph_handle* h);
 
+/* ------------------------------------------------------------ autotuning */
+
+/* Launch shapes of a PCISPH handle's list kernels, the single-scene form of
+ * the reference's launch autotuning (CLKernel::run, opencl/clkernel.cpp:
+ * 85-151, which times work-group sizes during the first launches). Kernels:
+ * SPH_TUNE_* below. A shape is threads per block (64, 128 or 256) and, for
+ * update_pressure and pressure_force, a fixed grid of blocks (0 = one thread
+ * per particle). Shapes change timing only, never results. SPH_ERR_STATE for
+ * a WCSPH handle. */
+#define SPH_TUNE_NORMALS 0
+#define SPH_TUNE_FORCES_PREDICT 1
+#define SPH_TUNE_UPDATE_PRESSURE 2
+#define SPH_TUNE_PRESSURE_FORCE 3
+/* Times every candidate shape of each kernel over `trials` steps from the
+ * current state, keeps the fastest and restores the state (positions and
+ * velocities). cache_path (may be NULL) names a text file of earlier results
+ * for the same device and scene size: when it holds them they are used, else
+ * the new ones are appended. Returns 1 if the cache supplied the shapes, 0 if
+ * they were measured. */
+int sph_autotune(sph_handle* h, const char* cache_path, int trials);
+int sph_get_launch(sph_handle* h, int kernel, int* block, int* grid);
+int sph_set_launch(sph_handle* h, int kernel, int block, int grid);
+
 /* ------------------------------------------------------------------ sort */
 
 /* clsort<cl_uint, cl_int> (opencl/algorithms/clsort.h:12-35): stable sort of
