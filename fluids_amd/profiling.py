@@ -6,6 +6,8 @@ Two summaries are produced from rocprofv3's CSV output:
   ``--kernel-trace`` (``*_kernel_trace.csv``), restricted to the dispatches at
   each kernel's largest grid, i.e. the launches over the whole particle set
   (the relaxation steps of a scene build run on fewer particles).
+* ``timeline``: per step, the span, the GPU busy time (union of dispatch
+  intervals) and the idle time between dispatches.
 * ``pmc``: per-kernel HBM traffic from two ``--pmc`` runs (FETCH_SIZE, then
   WRITE_SIZE, in KiB per dispatch), each scaled by what the same counter reads
   for a device copy of known size.
@@ -17,6 +19,7 @@ Usage::
     python -m fluids_amd.profiling pmc <fetch_dir> <write_dir> <copy_fetch_dir> <copy_write_dir> \
         --copy-bytes 268435456 [--tcc-dir <hit_miss_dir>] > traffic.md
     python -m fluids_amd.profiling stats <trace_dir or *kernel_stats.csv> > stats.md
+    python -m fluids_amd.profiling timeline <trace_dir> [--skip 10] > timeline.md
 
 ``bench.py`` reads the machine-readable ``traffic`` rows of a ``pmc`` summary
 (``read_traffic``) to fill its roofline's ``traffic`` field.
@@ -89,6 +92,45 @@ def trace_summary(trace_dir, traffic_file=None):
         out.append(f"| {k} | {grid[k]} | {len(v)} | {avg:.1f} | {min(v):.1f} | {max(v):.1f} | "
                    f"{sum(v) / 1e3:.3f} | {tb} | {bw} |")
     return "\n".join(out)
+
+
+def timeline_summary(trace_dir, marker="hash", skip=10):
+    """Step-by-step timeline of the solver's dispatches from a kernel trace:
+    a step starts at each full-grid dispatch of `marker` (the first kernel of
+    a step); the first `skip` steps (scene build, warm-up) are dropped. Per
+    step: span from the first dispatch's start to the next step's first
+    start, GPU busy time (the union of dispatch intervals), the idle time
+    between dispatches, and the dispatch count."""
+    rows = list(csv.DictReader(open(_csv(trace_dir, "kernel_trace.csv"))))
+    rows = [r for r in rows if "sph::" in r["Kernel_Name"]]
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    big = max(_grid(r) for r in rows if short_name(r["Kernel_Name"]) == marker)
+    starts = [i for i, r in enumerate(rows) if short_name(r["Kernel_Name"]) == marker and _grid(r) == big]
+    spans, busy, counts = [], [], []
+    for a, b in zip(starts[skip:], starts[skip + 1:]):
+        seg = rows[a:b]
+        t0, t1 = int(seg[0]["Start_Timestamp"]), int(rows[b]["Start_Timestamp"])
+        iv = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"])) for r in seg)
+        u, cs, ce = 0, iv[0][0], iv[0][1]
+        for s0, e0 in iv[1:]:
+            if s0 > ce:
+                u += ce - cs
+                cs, ce = s0, e0
+            else:
+                ce = max(ce, e0)
+        u += ce - cs
+        spans.append((t1 - t0) / 1e3)
+        busy.append(u / 1e3)
+        counts.append(len(seg))
+    if not spans:
+        return "no complete steps in the trace"
+    n = len(spans)
+    span, bsy = sum(spans) / n, sum(busy) / n
+    return "\n".join([
+        "| steps | dispatches / step | span us / step | GPU busy us / step | idle between dispatches us / step | idle per dispatch us |",
+        "|---|---|---|---|---|---|",
+        f"| {n} | {sum(counts) / n:.1f} | {span:.1f} | {bsy:.1f} | {span - bsy:.1f} | {(span - bsy) / (sum(counts) / n):.2f} |",
+    ])
 
 
 def _counter_means(d, counter):
@@ -193,6 +235,9 @@ def main(argv=None):
     p.add_argument("copy_write_dir")
     p.add_argument("--copy-bytes", type=int, required=True)
     p.add_argument("--tcc-dir", default=None, help="run with TCC_HIT_sum and TCC_MISS_sum, for L2 hit rates")
+    tl = sub.add_parser("timeline")
+    tl.add_argument("trace_dir")
+    tl.add_argument("--skip", type=int, default=10, help="steps dropped at the start (build, warm-up)")
     st = sub.add_parser("stats")
     st.add_argument("stats_csv", help="a *kernel_stats.csv or a directory holding one")
     a = ap.parse_args(argv)
@@ -200,6 +245,8 @@ def main(argv=None):
         copy_calibration(a.bytes)
     elif a.cmd == "trace":
         print(trace_summary(a.trace_dir, a.traffic))
+    elif a.cmd == "timeline":
+        print(timeline_summary(a.trace_dir, skip=a.skip))
     elif a.cmd == "stats":
         print(stats_summary(a.stats_csv))
     else:
