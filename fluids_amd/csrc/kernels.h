@@ -131,6 +131,8 @@ void launch_stats(int n, const int* nc, const int* bnc, const float4* pos, unsig
 
 // sort.hip: the body of sph_sort_pairs (throws sph::Error)
 void sort_pairs(uint32_t* keys, int32_t* values, int n, int key_bits, void* stream);
+// the body of sph_sort_pairs_noalloc (work arrays in the library's scratch)
+void sort_pairs_noalloc(uint32_t* keys, int32_t* values, int n, int key_bits, void* stream);
 
 // pcisph.hip
 void launch_forces_predict(int n, const float4* prho, const float4* vel_s, const float4* normal, const int* nl,

@@ -1,8 +1,11 @@
 // sph_sort_pairs: the library's stable key/value radix sort on device arrays
 // the caller owns, in place of the reference's clsort (opencl/algorithms/
 // clsort.h:12-35, a clogs::Radixsort over cl_mem keys and values), with the
-// kernels of grid.hip. Its ping-pong arrays and digit histograms come from
-// the stream-ordered allocator.
+// kernels of grid.hip. sph_sort_pairs takes its ping-pong arrays and digit
+// histograms from the stream-ordered allocator; sph_sort_pairs_noalloc takes
+// them from `scratch`, a fixed array in the library's device image, and
+// allocates nothing.
+#include <mutex>
 #include <string>
 
 #include "kernels.h"
@@ -17,6 +20,35 @@ namespace sph {
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 static size_t sort_scratch_bytes(int n) {
     return 2 * align256((size_t)n * 4) + align256(sort_hist_words(n) * 4);
+}
+
+// Scratch of the no-allocation sort, sized for SPH_SORT_NOALLOC_MAX pairs
+// (the key and value ping-pong halves and the digit histograms: 2048-key
+// tiles, at most 1024 digit counters per tile, plus the digit totals). Every
+// GPU the library runs on holds one copy until the process ends; all calls on
+// a GPU share it. In host code `scratch` only names the symbol: its address on
+// the current device comes from hipGetSymbolAddress (scratch_ptr).
+constexpr size_t NOALLOC_BYTES =
+    2 * ((size_t)SPH_SORT_NOALLOC_MAX * 4) + 4 * ((size_t)1024 * (SPH_SORT_NOALLOC_MAX / 2048) + 1024);
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current.
+static void* scratch_ptr() {
+    constexpr int MAX_DEVICES = 64;
+    static std::mutex mu;
+    static void* ptr[MAX_DEVICES] = {};
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= MAX_DEVICES) throw Error(SPH_ERR_HIP, "device index out of range");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!p) throw Error(SPH_ERR_HIP, "hipGetSymbolAddress gave no address for the sort scratch");
+        ptr[dev] = p;
+    }
+    return ptr[dev];
 }
 
 static void check_args(const uint32_t* keys, const int32_t* values, int n, int& key_bits) {
@@ -61,6 +93,20 @@ void sort_pairs(uint32_t* keys, int32_t* values, int n, int key_bits, void* stre
         throw;
     }
     hip_check(hipFreeAsync(work, s), "hipFreeAsync");
+}
+
+// The same sort with its work arrays in `scratch`: no allocation. The
+// histograms are written in full by every pass, so nothing in the scratch has
+// to start at zero; calls that overlap on one GPU share the scratch and race.
+void sort_pairs_noalloc(uint32_t* keys, int32_t* values, int n, int key_bits, void* stream) {
+    check_args(keys, values, n, key_bits);
+    if (n > SPH_SORT_NOALLOC_MAX)
+        throw Error(SPH_ERR_RANGE, "sph_sort_pairs_noalloc: " + std::to_string(n) +
+                                       " pairs exceed its scratch of SPH_SORT_NOALLOC_MAX = " +
+                                       std::to_string(SPH_SORT_NOALLOC_MAX) + " pairs");
+    if (sort_scratch_bytes(n) > NOALLOC_BYTES) throw Error(SPH_ERR_RANGE, "sort scratch too small");
+    void* work = scratch_ptr();
+    sort_in_place(keys, values, n, key_bits, work, static_cast<hipStream_t>(stream));
 }
 
 }  // namespace sph

@@ -981,3 +981,10 @@ extern "C" int sph_sort_pairs(uint32_t* keys, int32_t* values, int n, int key_bi
         return SPH_OK;
     });
 }
+
+extern "C" int sph_sort_pairs_noalloc(uint32_t* keys, int32_t* values, int n, int key_bits, void* stream) {
+    return guard([&] {
+        sort_pairs_noalloc(keys, values, n, key_bits, stream);
+        return SPH_OK;
+    });
+}

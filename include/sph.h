@@ -288,6 +288,18 @@ int sph_set_launch(sph_handle* h, int kernel, int block, int grid);
  * stream. n up to 2^27. */
 int sph_sort_pairs(uint32_t* keys, int32_t* values, int n, int key_bits, void* stream);
 
+/* No counterpart in the reference: sph_sort_pairs for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Same arguments, same result bit for bit; its work arrays
+ * live in a fixed array of the library's device image (about 10 MB, held by
+ * every GPU the library runs on until the process ends), so it calls no
+ * allocator. n is limited to SPH_SORT_NOALLOC_MAX (SPH_ERR_RANGE beyond).
+ * Every call on one GPU shares that array: calls that overlap in time on one
+ * GPU (other streams, threads or handles) race and must be serialised by the
+ * caller. */
+#define SPH_SORT_NOALLOC_MAX (1 << 20)
+int sph_sort_pairs_noalloc(uint32_t* keys, int32_t* values, int n, int key_bits, void* stream);
+
 /* -------------------------------------------------------------- samplers */
 /* Each returns the sample count; out (xyzw) receives up to cap samples. */
 
