@@ -850,7 +850,7 @@ void launch_normals(int n, const float4* prho, const int* nl, const int* nc, flo
 // ------------------------------------------------------------- step state
 __global__ void k_step_begin(DevState* st) {
     int t = threadIdx.x;
-    if (t < MAXERR_SLOTS) st->maxerr[t] = 0u;
+    for (int k = t; k < MAXERR_SLOTS * ERR_LANES; k += blockDim.x) (&st->maxerr[0][0])[k] = 0u;
     if (t == 0) {
         st->done_iter = 0;
         st->iterations = 0;

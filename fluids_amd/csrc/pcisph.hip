@@ -216,7 +216,7 @@ __device__ __forceinline__ void update_pressure_phase(int n, int iter, const flo
     if (threadIdx.x == 0) {
         uint32_t m = wmax[0];
         for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = wmax[w] > m ? wmax[w] : m;
-        atomicMax(&st->maxerr[iter & (MAXERR_SLOTS - 1)], m);
+        atomicMax(&st->maxerr[iter & (MAXERR_SLOTS - 1)][blockIdx.x & (ERR_LANES - 1)], m);
     }
 }
 
@@ -245,16 +245,21 @@ __device__ __forceinline__ void pressure_force_phase(
     const int* __restrict__ bnl, const int* __restrict__ bnc, const float4* __restrict__ bpos_s,
     const float* __restrict__ bphi_s, int has_b, const PciParams& P, float4* __restrict__ f_press,
     float4* __restrict__ pos_pred, float4* __restrict__ pos_out, float4* __restrict__ vel_out, DevState* st) {
-    float mx = __uint_as_float(
-        __hip_atomic_load(&st->maxerr[iter & (MAXERR_SLOTS - 1)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    bool last = iter >= P.max_iter || (iter >= P.min_iter && ((mx - P.rho0_host) / P.rho0_host < P.eta));
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        if (last) {
-            st->done_iter = iter;
-            st->iterations = iter;
-        }
-        st->maxerr[(iter + 1) & (MAXERR_SLOTS - 1)] = 0u;
+    // the maximum over the merged words: one per lane, then the wave's
+    // maximum (non-negative float bits order like the values)
+    uint32_t mbits = __hip_atomic_load(&st->maxerr[iter & (MAXERR_SLOTS - 1)][threadIdx.x & (ERR_LANES - 1)],
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int off = 32; off > 0; off >>= 1) {
+        uint32_t o = __shfl_xor(mbits, off);
+        mbits = o > mbits ? o : mbits;
     }
+    float mx = __uint_as_float(mbits);
+    bool last = iter >= P.max_iter || (iter >= P.min_iter && ((mx - P.rho0_host) / P.rho0_host < P.eta));
+    if (blockIdx.x == 0 && threadIdx.x == 0 && last) {
+        st->done_iter = iter;
+        st->iterations = iter;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < ERR_LANES) st->maxerr[(iter + 1) & (MAXERR_SLOTS - 1)][threadIdx.x] = 0u;
     const int stride = gridDim.x * blockDim.x;
     for (int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
     float4 ci = pc[i];

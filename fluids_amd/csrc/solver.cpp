@@ -740,7 +740,8 @@ void SimulationBase::stats(sph_step_stats& o) {
     o.boundary_list_overflows = (int)acc[3];
     o.nan_positions = (int)acc[4];
     int it = st.iterations > 0 ? st.iterations : 1;
-    uint32_t bits = st.maxerr[it & (MAXERR_SLOTS - 1)];
+    uint32_t bits = 0;
+    for (int l = 0; l < ERR_LANES; ++l) bits = std::max(bits, st.maxerr[it & (MAXERR_SLOTS - 1)][l]);
     float mx;
     std::memcpy(&mx, &bits, 4);
     o.max_density_error = s_.method == SPH_PCISPH ? mx : 0.0f;

@@ -25,6 +25,10 @@ namespace sph {
 
 constexpr int LIST_CAP = 50;      // NEIGH_LIST_MAX_LENGTH (kernels/common.h:4)
 constexpr int MAXERR_SLOTS = 16;  // ring of per-correction max density errors
+// Each correction's maximum is merged into ERR_LANES words (workgroup b into
+// word b mod ERR_LANES) and read back by one load per lane and a wave
+// maximum: thousands of workgroups then never queue on one address.
+constexpr int ERR_LANES = 64;
 constexpr int GATHER_BATCH = 8;  // neighbour loads issued together per thread
 constexpr int BOUND_BATCH = 4;   // boundary-neighbour loads issued together per thread
 // Largest fluid or boundary particle count of one scene (or one batch): the
@@ -69,7 +73,7 @@ struct GridDev {
 
 // Per-step device state written by the kernels.
 struct DevState {
-    unsigned int maxerr[MAXERR_SLOTS];  // float bits of max density error per correction (ring)
+    unsigned int maxerr[MAXERR_SLOTS][ERR_LANES];  // float bits of max density error per correction (ring)
     int done_iter;                      // correction at which the loop ended (0: running)
     int iterations;
     int out_of_grid;
