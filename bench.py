@@ -157,7 +157,7 @@ def run_batched(steps, warmup, plain_steps, count=512):
     scs = scenes.c4_batched(count=count)
     total = sum(sc.n_fluid for sc in scs)
     t0 = time.perf_counter()
-    b = scenes.build_batch(scs, autotune=True, trials=2)
+    b = scenes.build_batch(scs, autotune=True, trials=6)
     t_tune = time.perf_counter() - t0
     b.step(warmup)
     b.synchronize()
@@ -361,7 +361,7 @@ def main():
         achieved = alg / avg_s
         roof = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK, "traffic": read_traffic(a.traffic_file, dom),
-                "traffic_source": a.traffic_file, "alg_bytes_per_launch": alg,
+                "traffic_source": os.path.relpath(a.traffic_file, ROOT), "alg_bytes_per_launch": alg,
                 "avg_launch_us": 1e6 * avg_s, "timed_launches": launches}
 
     cpu = None

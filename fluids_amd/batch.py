@@ -46,7 +46,7 @@ class BatchSimulation:
     def finalize(self):
         check(lib().sph_batch_finalize(self._h))
 
-    def autotune(self, cache=None, trials=2) -> bool:
+    def autotune(self, cache=None, trials=4) -> bool:
         rc = check(lib().sph_batch_autotune(self._h, cache.encode() if cache else None, int(trials)))
         return rc == 1
 

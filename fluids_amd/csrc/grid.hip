@@ -524,8 +524,8 @@ __device__ __forceinline__ void batch_hits(V3 pi, int j0, int rem, const float4 
 
 // The walk over the 27 cells. Candidates are loaded WALK_BATCH at a time
 // through a buffer view, from one base offset per batch (the per-slot offset
-// is a scalar operand); loads past the cell's end read neighbours or zero and
-// are never used. The distance of every slot is computed unconditionally;
+// is a scalar operand), 12 bytes each (w is not needed); loads past the
+// cell's end read neighbours or zero and are never used. The distance of every slot is computed unconditionally;
 // the range, cap and radius tests only guard the list entry and the density
 // term, and the range and cap tests are skipped for batches where the whole
 // wave has a full batch left and room for it under the cap. Bit o of `cells`
@@ -556,7 +556,7 @@ __device__ __forceinline__ int walk_cells(V3 pi, int cx, int cy, int cz, const i
             float w[WALK_BATCH];
 #pragma unroll
             for (int u = 0; u < WALK_BATCH; ++u) {
-                q[u] = load4(rp, (uint32_t)j0 * 16u, 16u * u);
+                q[u] = load3(rp, (uint32_t)j0 * 16u, 16u * u);
                 w[u] = PHI ? load1(rw, (uint32_t)j0 * 4u, 4u * u) : 1.0f;
             }
             if (__all(rem >= WALK_BATCH && len <= LIST_CAP - WALK_BATCH))

@@ -307,7 +307,7 @@ def c4_batched(seed: int = 404, count: int = 512, n_min: int = 1000, n_max: int 
     return out
 
 
-def build_batch(scs: List[Scene], autotune: bool = True, cache: Optional[str] = None, trials: int = 2):
+def build_batch(scs: List[Scene], autotune: bool = True, cache: Optional[str] = None, trials: int = 4):
     """The scenes as one libsph batch (sph_batch_*), finalised and, unless
     told otherwise, autotuned."""
     from .batch import BatchSimulation
