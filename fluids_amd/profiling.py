@@ -4,8 +4,9 @@ Two summaries are produced from rocprofv3's CSV output:
 
 * ``trace``: per-kernel dispatch count and average / min / max duration from
   ``--kernel-trace`` (``*_kernel_trace.csv``), restricted to the dispatches at
-  each kernel's largest grid, i.e. the launches over the whole particle set
-  (the relaxation steps of a scene build run on fewer particles).
+  each kernel's most frequent launch shape, i.e. the run's steps over the
+  whole particle set (not the relaxation steps of a scene build nor the
+  launch autotuner's trial steps).
 * ``timeline``: per step, the span, the GPU busy time (union of dispatch
   intervals) and the idle time between dispatches.
 * ``pmc``: per-kernel HBM traffic from two ``--pmc`` runs (FETCH_SIZE, then
@@ -63,23 +64,52 @@ def _grid(r):
     return int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
 
 
-def _full_grid(rows, key):
-    """Keeps each kernel's dispatches at its largest grid size."""
-    big = {}
+def _wg(r):
+    for f in ("Workgroup_Size_X", "Workgroup_Size"):
+        if f in r and r[f]:
+            return int(r[f])
+    return 0
+
+
+def _full_grid(rows, key, min_count=16):
+    """Labels each dispatch by launch shape and keeps the run's own: per
+    kernel, shapes (grid, workgroup size) with at least `min_count`
+    dispatches (the scene build's relaxation steps and the launch
+    autotuner's trial steps have fewer); the shape with the most total time
+    keeps the kernel's name, others are named `kernel@grid/workgroup` (the
+    correction launches past the third, which return at once, are such a
+    shape). Returns (label, row) pairs."""
+    count = collections.Counter((key(r), _grid(r), _wg(r)) for r in rows)
+    total = collections.defaultdict(float)
     for r in rows:
-        k = key(r)
-        big[k] = max(big.get(k, 0), _grid(r))
-    return [r for r in rows if _grid(r) == big[key(r)]]
+        total[(key(r), _grid(r), _wg(r))] += _dur(r)
+    main = {}
+    for (k, g, w), c in count.items():
+        if c < min_count:
+            continue
+        if k not in main or total[(k, g, w)] > total[(k,) + main[k]]:
+            main[k] = (g, w)
+    out = []
+    for r in rows:
+        k, g, w = key(r), _grid(r), _wg(r)
+        if count[(k, g, w)] < min_count:
+            continue
+        out.append((k if main.get(k) == (g, w) else f"{k}@{g}/{w}", r))
+    return out
+
+
+def _dur(r):
+    if "End_Timestamp" in r:
+        return float(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    return 1.0
 
 
 def trace_summary(trace_dir, traffic_file=None):
     rows = list(csv.DictReader(open(_csv(trace_dir, "kernel_trace.csv"))))
     rows = [r for r in rows if "sph::" in r["Kernel_Name"]]
-    rows = _full_grid(rows, lambda r: short_name(r["Kernel_Name"]))
     dur = collections.defaultdict(list)
     grid = {}
-    for r in rows:
-        k = short_name(r["Kernel_Name"])
+    for k, r in _full_grid(rows, lambda r: short_name(r["Kernel_Name"])):
         dur[k].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
         grid[k] = _grid(r)
     out = ["| kernel | grid (threads) | dispatches | avg us | min us | max us | total ms | HBM bytes | HBM GB/s |",
