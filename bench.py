@@ -48,6 +48,8 @@ def kernel_bytes(name, K, Kb, n_it):
     n_it PCISPH corrections per step."""
     table = {
         "hash": 24.0,
+        "hash_moved": 24.0,
+        "resort_place": 16.0,  # the reference's sort: 16 B/p over all its passes
         "radix_scatter": 16.0 / 4,  # the reference's sort counts 16 B/p over all passes
         "radix_hist": 0.0,
         "radix_rowscan": 0.0,
@@ -249,6 +251,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-events", action="store_true", help="time without per-kernel events")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--full-sort", action="store_true",
+                    help="sort every key each step with the radix sort (SPH_FULL_SORT) instead of placing "
+                         "only the keys that changed cell")
     ap.add_argument("--json-out", default=None)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-scenes measurement")
     ap.add_argument("--batched-only", action="store_true", help="only the batched-scenes measurement")
@@ -282,6 +287,8 @@ def main():
     sc = scenes.c3_pcisph_throughput(a.n)
     if a.max_iter is not None:
         sc.settings["pcisph_max_iterations"] = a.max_iter
+    if a.full_sort:
+        sc.settings["flags"] = sc.settings.get("flags", 0) | 4  # SPH_FULL_SORT
     sim = scenes.build(sc)
     sim.synchronize()
     t_init = time.perf_counter() - t_init
@@ -393,6 +400,7 @@ def main():
         "config": {"workload": f"pcisph_throughput_{N // 1000}k", "fluid_particles": N, "boundary_particles": B,
                    "spacing_m": 0.005, "support_radius_m": 0.01, "corrections": round(n_it, 3),
                    "mean_fluid_neighbours": round(K, 3), "mean_boundary_neighbours": round(Kb, 3),
+                   "sort": "radix sort of every key" if a.full_sort else "moved keys placed into the previous order",
                    "counts": "means over the timed steps (replayed untimed from the same state; "
                              f"replay bit-identical: {replay_identical})"},
         "roofline": roof,

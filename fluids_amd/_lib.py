@@ -91,6 +91,7 @@ SIGNATURES = {
     "sph_set_launch": (I, [P, I, I, I]),
     "sph_sort_pairs": (I, [P, P, I, I, P]),
     "sph_sort_pairs_noalloc": (I, [P, P, I, I, P]),
+    "sph_sort_near": (I, [P, P, P, P, I, P]),
     "sph_set_profiling": (I, [P, I]),
     "sph_set_profiling_filter": (I, [P, C.c_char_p]),
     "sph_get_kernel_times": (I, [P, P, P, P, I]),

@@ -7,6 +7,7 @@
 // (stable key/value sort), opencl/algorithms/clshuffle.h:23-35 (gather) and
 // the density kernels that consume the lists (pcisph.cl:33-102,
 // wcsph.cl:32-79).
+#include <algorithm>
 #include <stdexcept>
 
 #include "batch.h"
@@ -20,20 +21,24 @@ namespace sph {
 // identity; the first sort pass generates it instead of reading it. Before
 // writing the new key, each thread clears the cell range named by the
 // previous step's sorted key at its index (prev may alias keys).
-__device__ __forceinline__ void hash_one(int i, const float4* __restrict__ pos, const GridDev& g, uint32_t* keys,
-                                         const uint32_t* prev, int2* __restrict__ R, DevState* st) {
-    if (R) {
-        uint32_t pk = prev[i];
-        int px = (int)compact_by_3(pk), py = (int)compact_by_3(pk >> 1), pz = (int)compact_by_3(pk >> 2);
-        long long b = py < g.cps ? box_index(px, py, pz, g) : -1;
-        if (b >= 0) R[b] = make_int2(0, 0);
-    }
-    float4 p = pos[i];
+__device__ __forceinline__ void clear_range(uint32_t pk, const GridDev& g, int2* __restrict__ R) {
+    int px = (int)compact_by_3(pk), py = (int)compact_by_3(pk >> 1), pz = (int)compact_by_3(pk >> 2);
+    long long b = py < g.cps ? box_index(px, py, pz, g) : -1;
+    if (b >= 0) R[b] = make_int2(0, 0);
+}
+
+__device__ __forceinline__ uint32_t cell_key(float4 p, const GridDev& g, DevState* st) {
     if (!(isfinite(p.x) && isfinite(p.y) && isfinite(p.z))) atomicAdd(&st->nonfinite, 1);
     int cx, cy, cz;
     grid_coords(p, g, cx, cy, cz);
-    keys[i] = morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
     if (box_index(cx, cy, cz, g) < 0) atomicAdd(&st->out_of_grid, 1);
+    return morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
+}
+
+__device__ __forceinline__ void hash_one(int i, const float4* __restrict__ pos, const GridDev& g, uint32_t* keys,
+                                         const uint32_t* prev, int2* __restrict__ R, DevState* st) {
+    if (R) clear_range(prev[i], g, R);
+    keys[i] = cell_key(pos[i], g, st);
 }
 
 __global__ __launch_bounds__(256) void k_hash(int n, const float4* __restrict__ pos, GridDev g,
@@ -303,6 +308,551 @@ void launch_hash(int n, const float4* pos, const GridDev& g, uint32_t* keys, con
     PROF_BEGIN(prof, "hash", s);
     hipLaunchKernelGGL(k_hash, dim3((n + 255) / 256), dim3(256), 0, s, n, pos, g, keys, prev, R, st);
     PROF_END(prof, s);
+}
+
+// ------------------------------------------- sort of a nearly sorted step
+// The same stable sort of (key, index) pairs, for keys k that differ little
+// from a non-decreasing array P of the same length (the solver's particles
+// are stored in the previous step's sorted order, so P is that step's sorted
+// keys and k[i] != P[i] only where particle i changed cell). With
+//   moved(i)  = k[i] != P[i],
+//   p_m       = #{x : (P[x], x) < (k[m], m)}   (insertion point of moved m
+//               among all pairs of P; P's pairs increase strictly with x),
+//   H[p]      = moved elements with insertion point p (0 <= p <= n),
+//   X         = exclusive scan of D[x] = H[x] - moved(x) over 0 .. n,
+// the sorted rank of an unmoved i is i + X[i] + H[i] (the unmoved before it
+// plus the moved below it), and that of a moved m is p_m + X[p_m] plus m's
+// rank by (key, index) among the moved elements with the same insertion
+// point (its group). Any non-decreasing P gives the exact stable sort; a P
+// close to the result makes the work proportional to the moved elements.
+//
+// The insertion point of a particle that moved into a cell occupied in P
+// comes from P's cell range (the range box the previous step's gather
+// wrote), checked against P itself; into an empty cell, from the range end
+// of the nearest occupied Morton predecessor among the next few keys, also
+// checked; otherwise (or without a range box) from a search of P through
+// its samples S (every 64th key) and S2 (every 4096th), which every sort
+// writes with its output.
+//
+// Kernels:
+//   k_resort_flag    keys, moved mask, insertion points, per-tile sums of D
+//                    and of the moved count; the moved lanes of a wave with
+//                    one insertion point (a subgroup) reserve their slots
+//                    with one atomic and take them in (key, index) order
+//   k_resort_place   each tile's prefix from the tile sums before it; the
+//                    scans of D and of the moved flags inside the tile; the
+//                    unmoved elements to their ranks; each group's base rank
+//                    and size; the moved list
+//   k_resort_moved   groups that came from one subgroup straight to their
+//                    ranks; the others gathered by slot; the cells the
+//                    moved elements left cleared in the range box
+//   k_resort_groups  gathered groups: up to RESORT_SMALL_GROUP ranked by
+//                    counting, larger ones sorted by a workgroup each
+// Counters return to zero by the end.
+constexpr int RESORT_MIN_TILE_SHIFT = 10;  // scan tiles of 1024 positions (256 threads x 4) ...
+constexpr int RESORT_MAX_TILES = 4096;     // ... or larger, so that there are at most this many
+constexpr int RESORT_SMALL_GROUP = 64;     // groups ranked by their list up to this size
+constexpr int RESORT_LDS_GROUP = 4096;     // larger groups sort in LDS up to this size, else in place
+constexpr int RESORT_LIST_BLOCKS = 512;    // workgroups that walk the list of moved elements
+constexpr int RESORT_BIG_BLOCKS = 64;      // workgroups that sort the groups of more than RESORT_SMALL_GROUP
+constexpr int RESORT_PRED = 4;             // Morton predecessors tried for a move into an empty cell
+// The tile sums are updated with atomics from every wave; atomics on one
+// cache line queue up behind each other, so each tile's counter has a
+// 128-byte line to itself.
+constexpr int RESORT_PAD = 32;
+
+static int resort_tile_shift(int n) {
+    int sh = RESORT_MIN_TILE_SHIFT;
+    while ((n >> sh) + 1 > RESORT_MAX_TILES) ++sh;
+    return sh;
+}
+static int resort_tiles(int n) { return (n >> resort_tile_shift(n)) + 1; }  // positions 0 .. n
+
+ResortSizes resort_sizes(int n) {
+    ResortSizes z;
+    const size_t N = (size_t)(n > 0 ? n : 1);
+    z.tiles = (size_t)resort_tiles(n);
+    z.padded = z.tiles << resort_tile_shift(n);
+    z.mask_words = (N + 63) / 64;
+    z.elems = N;
+    z.big = N / (RESORT_SMALL_GROUP + 1) + 1;
+    z.s1 = N / 64 + 1;
+    z.s2 = N / 4096 + 1;
+    return z;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Byte sizes of the work arrays in the order resort_carve lays them out.
+constexpr int RESORT_PARTS = 14;
+static void resort_parts(int n, size_t out[RESORT_PARTS]) {
+    const ResortSizes z = resort_sizes(n);
+    const size_t sizes[RESORT_PARTS] = {z.mask_words * 8, z.padded * 4, z.padded * 8, z.tiles * 4 * RESORT_PAD, z.tiles * 4 * RESORT_PAD,
+                                        z.elems * 16,     z.elems * 4,  z.elems * 4,  z.elems * 4, z.big * 4,
+                                        4,                4,            z.s1 * 4,     z.s2 * 4};
+    for (int k = 0; k < RESORT_PARTS; ++k) out[k] = sizes[k];
+}
+
+size_t resort_bytes(int n) {
+    size_t parts[RESORT_PARTS], total = 0;
+    resort_parts(n, parts);
+    for (size_t b : parts) total += align256(b);
+    return total;
+}
+
+ResortWork resort_carve(void* block, int n) {
+    size_t parts[RESORT_PARTS];
+    resort_parts(n, parts);
+    unsigned char* q = static_cast<unsigned char*>(block);
+    void* a[RESORT_PARTS];
+    for (int k = 0; k < RESORT_PARTS; ++k) {
+        a[k] = q;
+        q += align256(parts[k]);
+    }
+    ResortWork w;
+    w.moved = static_cast<unsigned long long*>(a[0]);
+    w.H = static_cast<int*>(a[1]);
+    w.grp = static_cast<int2*>(a[2]);
+    w.T = static_cast<int*>(a[3]);
+    w.TF = static_cast<int*>(a[4]);
+    w.rec = static_cast<int4*>(a[5]);
+    w.mlist = static_cast<int*>(a[6]);
+    w.tk = static_cast<uint32_t*>(a[7]);
+    w.ti = static_cast<int*>(a[8]);
+    w.big = static_cast<int*>(a[9]);
+    w.nbig = static_cast<int*>(a[10]);
+    w.nmoved = static_cast<int*>(a[11]);
+    w.S = static_cast<uint32_t*>(a[12]);
+    w.S2 = static_cast<uint32_t*>(a[13]);
+    return w;
+}
+
+void resort_clear(const ResortWork& w, int n, hipStream_t s) {
+    const ResortSizes z = resort_sizes(n);
+    launch_fill_u32(w.H, z.padded, 0u, s);
+    launch_fill_u32(w.T, z.tiles * RESORT_PAD, 0u, s);
+    launch_fill_u32(w.TF, z.tiles * RESORT_PAD, 0u, s);
+}
+
+// Every sort writes the samples of its output (rank r, key k).
+__device__ __forceinline__ void put_sample(const ResortWork& w, int r, uint32_t k) {
+    if ((r & 63) == 0) w.S[r >> 6] = k;
+    if ((r & 4095) == 0) w.S2[r >> 12] = k;
+}
+
+__global__ __launch_bounds__(256) void k_resort_sample(const uint32_t* __restrict__ P, int n, ResortWork w) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j * 64 < n) put_sample(w, j * 64, P[j * 64]);
+}
+
+void launch_resort_samples(const uint32_t* sorted, int n, const ResortWork& w, hipStream_t s) {
+    if (n <= 0) return;
+    const int m = (n + 63) / 64;
+    hipLaunchKernelGGL(k_resort_sample, dim3((m + 255) / 256), dim3(256), 0, s, sorted, n, w);
+}
+
+// (v, x) < (k, m) for the pair at position x of P holding v
+__device__ __forceinline__ bool below(uint32_t v, int x, uint32_t k, int m) { return v < k || (v == k && x < m); }
+
+// Insertion point of (k, m) by search: the first sample of S2 that is not
+// below, then of S inside that stretch, then of P inside the 64 positions
+// left. P's pairs below (k, m) form a prefix, so each level is a binary
+// search; the samples are small enough to stay in the caches.
+__device__ int insertion_search(const uint32_t* __restrict__ P, const ResortWork& w, int n, uint32_t k, int m) {
+    int lo = 0, hi = (n + 4095) >> 12;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (below(w.S2[mid], mid << 12, k, m)) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo == 0) return 0;
+    int jlo = ((lo - 1) << 6) + 1, jhi = min(lo << 6, (n + 63) >> 6);
+    while (jlo < jhi) {
+        const int mid = (jlo + jhi) >> 1;
+        if (below(w.S[mid], mid << 6, k, m)) jlo = mid + 1;
+        else jhi = mid;
+    }
+    int xlo = ((jlo - 1) << 6) + 1, xhi = min(jlo << 6, n);
+    while (xlo < xhi) {
+        const int mid = (xlo + xhi) >> 1;
+        if (below(P[mid], mid, k, m)) xlo = mid + 1;
+        else xhi = mid;
+    }
+    return xlo;
+}
+
+// P's range of cell key k from the range box ((0, 0) when the cell is
+// outside the box or empty).
+__device__ __forceinline__ int2 key_range(uint32_t k, const int2* __restrict__ R, const GridDev& g) {
+    const int x = (int)compact_by_3(k), y = (int)compact_by_3(k >> 1), z = (int)compact_by_3(k >> 2);
+    const long long b = y < g.cps ? box_index(x, y, z, g) : -1;
+    return b >= 0 ? R[b] : make_int2(0, 0);
+}
+
+// Insertion point of (k, m): from P's range of k when P holds k (the range
+// is accepted only if P confirms both ends), else from the end of the
+// nearest occupied predecessor's range when P confirms that k falls right
+// there, else by search. The confirming reads are issued together (indices
+// clamped into P; the tests only use the ones that apply).
+__device__ int insertion_point(const uint32_t* __restrict__ P, const ResortWork& w, int n, uint32_t k, int m,
+                               const int2* __restrict__ R, const GridDev& g) {
+    if (R) {
+        const int2 r = key_range(k, R, g);
+        if (r.y > r.x && r.x >= 0 && r.y <= n) {
+            const uint32_t a0 = P[r.x > 0 ? r.x - 1 : 0], a1 = P[r.x], a2 = P[r.y - 1], a3 = P[r.y < n ? r.y : n - 1];
+            if ((a1 == k) & (a2 == k) & ((r.x == 0) | (a0 < k)) & ((r.y == n) | (a3 > k)))
+                return m < r.x ? r.x : (m > r.y ? r.y : m);
+        } else if (r.y <= r.x) {
+            int2 q[RESORT_PRED];
+#pragma unroll
+            for (int t = 0; t < RESORT_PRED; ++t)
+                q[t] = (uint32_t)(t + 1) <= k ? key_range(k - (uint32_t)(t + 1), R, g) : make_int2(0, 0);
+            int p = -1;
+#pragma unroll
+            for (int t = RESORT_PRED - 1; t >= 0; --t)
+                if (q[t].y > q[t].x) p = q[t].y;  // the nearest occupied predecessor wins
+            if (p > 0 && p <= n) {
+                const uint32_t a0 = P[p - 1], a1 = P[p < n ? p : n - 1];
+                if ((a0 < k) & ((p == n) | (a1 > k))) return p;
+            }
+        }
+    }
+    return insertion_search(P, w, n, k, m);
+}
+
+// HASH: the keys are the cells of pos (compute_hashes, with k_hash's
+// counters) and are written to keys; otherwise keys holds them. R (may be
+// null): the range box of P's cells.
+template <bool HASH>
+__global__ __launch_bounds__(256) void k_resort_flag(int n, int tile_shift, const float4* __restrict__ pos, GridDev g,
+                                                     uint32_t* __restrict__ keys, const uint32_t* __restrict__ P,
+                                                     const int2* __restrict__ R, DevState* st, ResortWork w) {
+    const int i = (HASH ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x) * blockDim.x + threadIdx.x;
+    bool moved = false;
+    uint32_t k = 0, pk = 0;
+    if (i < n) {
+        pk = P[i];
+        if (HASH) {
+            k = cell_key(pos[i], g, st);
+            keys[i] = k;
+        } else {
+            k = keys[i];
+        }
+        moved = k != pk;
+    }
+    const uint64_t b = __ballot(moved);
+    const int lane = threadIdx.x & 63;
+    const int i0 = i - lane;  // blocks are whole waves, so i0 is 64-aligned
+    if (lane == 0 && i0 < n) w.moved[i0 >> 6] = b;
+    if (!b) return;
+    const int p = moved ? insertion_point(P, w, n, k, i, R, g) : -1;
+    // The wave's moved lanes with the same insertion point form a subgroup:
+    // its first lane (the leader) reserves the subgroup's slots with one
+    // atomic, and each member's slot is the leader's base plus the member's
+    // rank by (key, index) inside the subgroup (lanes are in index order).
+    int leader = lane, cnt = 0, rank = 0;
+    for (uint64_t todo = b; todo;) {  // wave-uniform
+        const int l = __ffsll((long long)todo) - 1;
+        const int pl = __shfl(p, l, 64);
+        const uint32_t kl = __shfl(k, l, 64);
+        const bool mine = moved && p == pl;
+        const uint64_t same = __ballot(mine);
+        const bool one_key = __ballot(mine && k != kl) == 0ull;
+        if (mine) {
+            leader = l;
+            cnt = __popcll(same);
+            if (one_key) {
+                rank = __popcll(same & ((1ull << lane) - 1ull));
+            } else {
+                rank = 0;
+                for (uint64_t m = same; m;) {
+                    const int j = __ffsll((long long)m) - 1;
+                    m &= m - 1;
+                    const uint32_t kj = __shfl(k, j, 64);  // every lane of `same` takes part
+                    rank += (kj < k || (kj == k && j < lane)) ? 1 : 0;
+                }
+            }
+        }
+        todo &= ~same;
+    }
+    int base = 0;
+    if (moved && leader == lane) base = atomicAdd(&w.H[p], cnt);
+    base = __shfl(base, leader, 64);
+    // tile sums: the wave's own tile gets -(moved lanes) plus its subgroups
+    // that insert there, in one atomic; subgroups inserting elsewhere add
+    // their own counts
+    const int own = i0 >> tile_shift;
+    const bool lead = moved && leader == lane;
+    const bool home = lead && (p >> tile_shift) == own;
+    int d = home ? cnt : 0;
+    for (int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off, 64);
+    if (lane == 0) {
+        atomicAdd(&w.T[own * RESORT_PAD], d - __popcll(b));
+        atomicAdd(&w.TF[own * RESORT_PAD], __popcll(b));
+    }
+    if (lead && !home) atomicAdd(&w.T[(p >> tile_shift) * RESORT_PAD], cnt);
+    if (moved) {
+        // z: the subgroup's size when it took the group's first slots, else 0
+        // (a group whose size equals that number came from one wave, ranked)
+        w.rec[i] = make_int4(p, base + rank, base == 0 ? cnt : 0, (int)pk);
+    }
+}
+
+// Sums of the padded counters a[t * RESORT_PAD] and b[t * RESORT_PAD] over
+// t < m (every thread of the workgroup gets them).
+__device__ __forceinline__ int2 block_sum2(const int* __restrict__ a, const int* __restrict__ b, int m,
+                                           int2* part) {
+    int x = 0, y = 0;
+    for (int t = threadIdx.x; t < m; t += blockDim.x) {
+        x += a[t * RESORT_PAD];
+        y += b[t * RESORT_PAD];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        x += __shfl_xor(x, off, 64);
+        y += __shfl_xor(y, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = make_int2(x, y);
+    __syncthreads();
+    int2 r = make_int2(0, 0);
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) {
+        r.x += part[q].x;
+        r.y += part[q].y;
+    }
+    __syncthreads();
+    return r;
+}
+
+// One tile of 2^tile_shift positions, `per` consecutive per thread (a
+// multiple of 4): the unmoved elements written to their ranks, the moved
+// ones to the moved list (index order), and for each group its base rank
+// p + X[p], size and list head (grp[p]), with H and the list head reset.
+// H and the list heads are padded to whole tiles.
+__global__ __launch_bounds__(256) void k_resort_place(int n, int tile_shift, int ntiles,
+                                                      const uint32_t* __restrict__ keys, ResortWork w,
+                                                      uint32_t* __restrict__ sk, int* __restrict__ perm) {
+    __shared__ uint32_t wtot[4];
+    __shared__ int2 part[4];
+    const int per = (1 << tile_shift) >> 8;
+    const int base = (blockIdx.x << tile_shift) + threadIdx.x * per;
+    const int2 before = block_sum2(w.T, w.TF, blockIdx.x, part);
+    int s = 0, nf = 0;
+    for (int c = 0; c < per; c += 4) {
+        const int4 hv = *reinterpret_cast<const int4*>(w.H + base + c);
+        const uint32_t fl = base + c < n ? (uint32_t)(w.moved[(base + c) >> 6] >> ((base + c) & 63)) & 0xfu : 0u;
+        s += hv.x + hv.y + hv.z + hv.w - __popc(fl);
+        nf += __popc(fl);
+    }
+    const int incl = (int)block_incl_scan_256((uint32_t)s, wtot);
+    const int incl_f = (int)block_incl_scan_256((uint32_t)nf, wtot);
+    int x = before.x + incl - s;
+    int f = before.y + incl_f - nf;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *w.nbig = 0;
+    if (blockIdx.x == ntiles - 1 && threadIdx.x == 255) *w.nmoved = before.y + incl_f;
+    for (int c = 0; c < per; c += 4) {
+        const int4 hv = *reinterpret_cast<const int4*>(w.H + base + c);
+        const int h[4] = {hv.x, hv.y, hv.z, hv.w};
+        const uint32_t fl = base + c < n ? (uint32_t)(w.moved[(base + c) >> 6] >> ((base + c) & 63)) & 0xfu : 0u;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int idx = base + c + u;
+            const bool mv = (fl >> u) & 1u;
+            if (h[u] > 0) {
+                w.grp[idx] = make_int2(idx + x, h[u]);
+                w.H[idx] = 0;
+            }
+            if (mv) {
+                w.mlist[f] = idx;
+                ++f;
+            } else if (idx < n) {
+                const int r = idx + x + h[u];
+                if ((unsigned)r < (unsigned)n) {  // always, for consistent work arrays
+                    const uint32_t k = keys[idx];
+                    sk[r] = k;
+                    perm[r] = idx;
+                    put_sample(w, r, k);
+                }
+            }
+            x += h[u] - (int)mv;
+        }
+    }
+}
+
+__device__ __forceinline__ bool pair_less(uint32_t ka, int ia, uint32_t kb, int ib) {
+    return ka < kb || (ka == kb && ia < ib);
+}
+
+// Ascending bitonic network on G = 2^m positions (the form in which every
+// comparator puts the smaller pair at the lower index: the first merge step
+// of each size pairs a with a ^ (size - 1), the others a with a + stride).
+// Positions at or past g hold +infinity: their comparators are skipped.
+template <class Load, class Store>
+__device__ void bitonic_pairs(int g, int G, Load load, Store store) {
+    for (int size = 2; size <= G; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int e = threadIdx.x; e < (G >> 1); e += blockDim.x) {
+                const int a = 2 * e - (e & (stride - 1));
+                const int c = stride == (size >> 1) ? (a ^ (size - 1)) : a + stride;
+                if (c >= g) continue;
+                uint32_t ka, kc;
+                int ia, ic;
+                load(a, ka, ia);
+                load(c, kc, ic);
+                if (pair_less(kc, ic, ka, ia)) {
+                    store(a, kc, ic);
+                    store(c, ka, ia);
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// One large group (gathered at tk/ti[b .. b + g)) sorted by the workgroup,
+// in LDS when it fits, and written to its ranks.
+__device__ void sort_group(int b, int g, const ResortWork& w, uint32_t* __restrict__ sk, int* __restrict__ perm,
+                           uint32_t* lk, int* li) {
+    uint32_t* gk = w.tk + b;
+    int* gi = w.ti + b;
+    int G = 1;
+    while (G < g) G <<= 1;
+    if (G <= RESORT_LDS_GROUP) {
+        for (int u = threadIdx.x; u < g; u += blockDim.x) {
+            lk[u] = gk[u];
+            li[u] = gi[u];
+        }
+        __syncthreads();
+        bitonic_pairs(g, G, [&](int a, uint32_t& k, int& v) { k = lk[a]; v = li[a]; },
+                      [&](int a, uint32_t k, int v) { lk[a] = k; li[a] = v; });
+        for (int u = threadIdx.x; u < g; u += blockDim.x) {
+            sk[b + u] = lk[u];
+            perm[b + u] = li[u];
+            put_sample(w, b + u, lk[u]);
+        }
+    } else {
+        bitonic_pairs(g, G, [&](int a, uint32_t& k, int& v) { k = gk[a]; v = gi[a]; },
+                      [&](int a, uint32_t k, int v) { gk[a] = k; gi[a] = v; });
+        for (int u = threadIdx.x; u < g; u += blockDim.x) {
+            sk[b + u] = gk[u];
+            perm[b + u] = gi[u];
+            put_sample(w, b + u, gk[u]);
+        }
+    }
+    __syncthreads();  // the LDS is read by every thread before the next group overwrites it
+}
+
+// A group came from one wave's subgroup when that subgroup took its first
+// slots and is as large as the group: its slots are then the ranks.
+__device__ __forceinline__ bool ranked_group(int4 rc, int2 e) { return rc.z == e.y; }
+
+// Moved elements (the list, walked by a fixed grid): elements of groups
+// that came from one wave go straight to their ranks; the others are
+// gathered at base + slot for k_resort_groups (and a group of more than
+// RESORT_SMALL_GROUP is listed for a workgroup sort). With a range box, each
+// moved element clears the range of the cell it left (cells that keep
+// particles are rewritten by the gather that follows). Also returns the tile
+// sums to zero.
+__global__ __launch_bounds__(256) void k_resort_moved(int n, int ntiles, int big_cap, const uint32_t* __restrict__ keys,
+                                                      ResortWork w, uint32_t* __restrict__ sk,
+                                                      int* __restrict__ perm, int2* __restrict__ R, GridDev g) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+    for (int t = gid; t < ntiles; t += stride) {
+        w.T[t * RESORT_PAD] = 0;
+        w.TF[t * RESORT_PAD] = 0;
+    }
+    const int nm = *w.nmoved;
+    for (int q = gid; q < nm; q += stride) {
+        const int i = w.mlist[q];
+        if ((unsigned)i >= (unsigned)n) continue;  // never, for consistent work arrays
+        const int4 rc = w.rec[i];  // (insertion point, slot, first subgroup's size or 0, key in the hint)
+        if (R) clear_range((uint32_t)rc.w, g, R);
+        const int2 e = w.grp[rc.x];
+        const int b = e.x, gs = e.y;
+        if (gs < 1 || b < 0 || b > n - gs || (unsigned)rc.y >= (unsigned)gs) continue;
+        const uint32_t k = keys[i];
+        if (ranked_group(rc, e)) {
+            sk[b + rc.y] = k;
+            perm[b + rc.y] = i;
+            put_sample(w, b + rc.y, k);
+            continue;
+        }
+        w.tk[b + rc.y] = k;
+        w.ti[b + rc.y] = i;
+        if (gs > RESORT_SMALL_GROUP && rc.y == 0) {
+            const int at = atomicAdd(w.nbig, 1);
+            if (at < big_cap) w.big[at] = rc.x;
+        }
+    }
+}
+
+// The gathered groups: up to RESORT_SMALL_GROUP, each member counts the
+// members below it (the first list_blocks workgroups walk the moved list);
+// larger ones are sorted by one workgroup each (the other workgroups).
+__global__ __launch_bounds__(256) void k_resort_groups(int n, int list_blocks, int big_cap,
+                                                       const uint32_t* __restrict__ keys, ResortWork w,
+                                                       uint32_t* __restrict__ sk, int* __restrict__ perm) {
+    __shared__ uint32_t lk[RESORT_LDS_GROUP];
+    __shared__ int li[RESORT_LDS_GROUP];
+    if ((int)blockIdx.x < list_blocks) {
+        const int nm = *w.nmoved;
+        for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nm; q += list_blocks * blockDim.x) {
+            const int i = w.mlist[q];
+            if ((unsigned)i >= (unsigned)n) continue;
+            const int4 rc = w.rec[i];
+            const int2 e = w.grp[rc.x];
+            const int b = e.x, gs = e.y;
+            if (gs < 1 || b < 0 || b > n - gs || ranked_group(rc, e) || gs > RESORT_SMALL_GROUP) continue;
+            const uint32_t k = keys[i];
+            int r = 0;
+            for (int u = b; u < b + gs; ++u) r += pair_less(w.tk[u], w.ti[u], k, i) ? 1 : 0;
+            sk[b + r] = k;
+            perm[b + r] = i;
+            put_sample(w, b + r, k);
+        }
+        return;
+    }
+    const int nb = min(*w.nbig, big_cap);
+    for (int q = blockIdx.x - list_blocks; q < nb; q += gridDim.x - list_blocks) {
+        const int2 e = w.grp[w.big[q]];
+        if (e.y < 1 || e.x < 0 || e.x > n - e.y) continue;  // uniform over the workgroup
+        sort_group(e.x, e.y, w, sk, perm, lk, li);
+    }
+}
+
+// keys_out/perm: the sorted keys and the permutation. P (read) and keys_out
+// may be the same array: P is last read by k_resort_flag.
+static void resort_rest(int n, const uint32_t* keys, uint32_t* keys_out, int* perm, const ResortWork& w, int2* R,
+                        const GridDev& g, hipStream_t s, Profiler* prof) {
+    const int sh = resort_tile_shift(n), ntiles = resort_tiles(n);
+    const int list_blocks = std::min(RESORT_LIST_BLOCKS, (n + 255) / 256);
+    PROF_BEGIN(prof, "resort_place", s);
+    hipLaunchKernelGGL(k_resort_place, dim3(ntiles), dim3(256), 0, s, n, sh, ntiles, keys, w, keys_out, perm);
+    PROF_END(prof, s);
+    const int big_cap = (int)resort_sizes(n).big;
+    PROF_BEGIN(prof, "resort_moved", s);
+    hipLaunchKernelGGL(k_resort_moved, dim3(list_blocks), dim3(256), 0, s, n, ntiles, big_cap, keys, w, keys_out, perm,
+                       R, g);
+    PROF_END(prof, s);
+    PROF_BEGIN(prof, "resort_groups", s);
+    hipLaunchKernelGGL(k_resort_groups, dim3(list_blocks + RESORT_BIG_BLOCKS), dim3(256), 0, s, n, list_blocks,
+                       big_cap, keys, w, keys_out, perm);
+    PROF_END(prof, s);
+}
+
+void launch_hash_resort(int n, const float4* pos, const GridDev& g, uint32_t* keys, uint32_t* sorted, int* perm,
+                        int2* R, DevState* st, const ResortWork& w, hipStream_t s, Profiler* prof) {
+    if (n <= 0) return;
+    PROF_BEGIN(prof, "hash_moved", s);
+    hipLaunchKernelGGL(k_resort_flag<true>, dim3((n + 255) / 256), dim3(256), 0, s, n, resort_tile_shift(n), pos, g,
+                       keys, sorted, R, st, w);
+    PROF_END(prof, s);
+    resort_rest(n, keys, sorted, perm, w, R, g, s, prof);
+}
+
+void launch_resort(int n, const uint32_t* keys, uint32_t* sorted, int* perm, const ResortWork& w, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_resort_flag<false>, dim3((n + 255) / 256), dim3(256), 0, s, n, resort_tile_shift(n), nullptr,
+                       GridDev{}, const_cast<uint32_t*>(keys), sorted, nullptr, nullptr, w);
+    resort_rest(n, keys, sorted, perm, w, nullptr, GridDev{}, s, nullptr);
 }
 
 // ------------------------------------------------------ gather + ranges

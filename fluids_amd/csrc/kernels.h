@@ -98,6 +98,49 @@ int sort_passes(int key_bits, int sid_bits);
 size_t sort_hist_words(int n);
 void launch_hash(int n, const float4* pos, const GridDev& g, uint32_t* keys, const uint32_t* prev, int2* R,
                  DevState* st, hipStream_t s, Profiler* prof);
+
+// Work arrays of the sort of a nearly sorted key array (grid.hip). One
+// device block of resort_bytes(n) bytes holds them all: resort_carve lays
+// them out, resort_clear zeroes the counters (H, T, TF); every sort leaves
+// them so.
+struct ResortWork {
+    unsigned long long* moved;  // mask_words: bit i set when key i differs from the hint
+    int* H;                     // padded: moved elements per insertion point
+    int2* grp;                  // padded: (base rank, size) of each insertion point's group
+    int* T;                     // tiles x 32 (one counter per 128-byte line): per-tile sums of H - moved
+    int* TF;                    // tiles x 32: per-tile moved counts
+    int4* rec;                  // elems: per moved element (insertion point, slot, size of its wave's
+                                //        subgroup if that took the group's first slots else 0, key in the hint)
+    int* mlist;                 // elems: the moved elements in index order
+    uint32_t* tk;               // elems: groups gathered at base + slot (keys)
+    int* ti;                    // elems: (indices)
+    int* big;                   // big: insertion points of the groups sorted by a workgroup
+    int* nbig;                  // 1
+    int* nmoved;                // 1
+    uint32_t* S;                // s1: every 64th key of the last sort's output
+    uint32_t* S2;               // s2: every 4096th
+};
+struct ResortSizes {
+    size_t tiles = 0, padded = 0, mask_words = 0, elems = 0, big = 0, s1 = 0, s2 = 0;
+};
+ResortSizes resort_sizes(int n);
+size_t resort_bytes(int n);
+ResortWork resort_carve(void* block, int n);
+void resort_clear(const ResortWork& w, int n, hipStream_t s);
+// Writes the samples (w.S, w.S2) of a sorted key array produced by another
+// sort, so that it can serve as the next hint.
+void launch_resort_samples(const uint32_t* sorted, int n, const ResortWork& w, hipStream_t s);
+// The step's keys (hashed from pos into keys, with k_hash's counters) sorted
+// with `sorted` as the hint: on entry the previous sorted keys (any
+// non-decreasing array of n keys whose samples w holds), on exit the sorted
+// keys (and their samples); perm receives the stable order. R (may be null):
+// the range box of the hint's cells, used to place moved keys and left with
+// the ranges of the cells that lost all their particles cleared, as
+// launch_hash clears them. Equal to launch_hash + launch_radix_sort.
+void launch_hash_resort(int n, const float4* pos, const GridDev& g, uint32_t* keys, uint32_t* sorted, int* perm,
+                        int2* R, DevState* st, const ResortWork& w, hipStream_t s, Profiler* prof);
+// The same sort of given keys.
+void launch_resort(int n, const uint32_t* keys, uint32_t* sorted, int* perm, const ResortWork& w, hipStream_t s);
 // Stable sort of n (key, index) pairs by the low key_bits of the keys; with
 // sid (scene id of each index, < 2^sid_bits) a final pass orders the result
 // by scene, so each scene's contiguous segment is sorted by its own keys.
@@ -133,6 +176,9 @@ void launch_stats(int n, const int* nc, const int* bnc, const float4* pos, unsig
 void sort_pairs(uint32_t* keys, int32_t* values, int n, int key_bits, void* stream);
 // the body of sph_sort_pairs_noalloc (work arrays in the library's scratch)
 void sort_pairs_noalloc(uint32_t* keys, int32_t* values, int n, int key_bits, void* stream);
+// the body of sph_sort_near
+void sort_near(const uint32_t* hint, const uint32_t* keys, uint32_t* sorted_keys, int32_t* perm, int n,
+               void* stream);
 
 // pcisph.hip
 void launch_forces_predict(int n, const float4* prho, const float4* vel_s, const float4* normal, const int* nl,

@@ -502,6 +502,10 @@ void SimulationBase::alloc_fluid(int n) {
     d_density.resize(N); d_pressure.resize(N); d_err.resize(N);
     d_keys_a.resize(N); d_keys_b.resize(N); d_vals_a.resize(N); d_vals_b.resize(N);
     d_hist.resize(sort_hist_words(n));
+    sorted_hint_ = false;
+    d_resort.resize(resort_bytes(n));
+    resort_ = resort_carve(d_resort.get(), n);
+    resort_clear(resort_, n, stream_);
     d_nl.resize(N * LIST_CAP); d_nc.resize(N); d_bnl.resize(N * LIST_CAP); d_bnc.resize(N);
     if (s_.method == SPH_WCSPH) {
         d_vh.resize(N); d_vh_s.resize(N); d_accel.resize(N);
@@ -574,14 +578,23 @@ void SimulationBase::require_ready() const {
 
 // hash -> stable sort -> gather + ranges (pcisphsimluation.cpp:79-97)
 void SimulationBase::neighbour_search() {
-    const int passes = sort_passes(grid_.key_bits);
-    uint32_t* hash_out = (passes % 2) ? d_keys_b.get() : d_keys_a.get();
     // the previous step's sorted keys (d_keys_a) name the cells to clear
-    launch_hash(n_, d_pos.get(), grid_.dev, hash_out, d_keys_a.get(), fresh_ranges_ ? nullptr : d_ranges.get(),
-                d_state.get(), stream_, &prof_);
+    int2* clear = fresh_ranges_ ? nullptr : d_ranges.get();
+    if (sorted_hint_ && !(s_.flags & SPH_FULL_SORT)) {
+        // particles are stored in the previous step's sorted order: only the
+        // keys that changed cell are placed; the result is the same sort
+        launch_hash_resort(n_, d_pos.get(), grid_.dev, d_keys_b.get(), d_keys_a.get(), d_vals_a.get(), clear,
+                           d_state.get(), resort_, stream_, &prof_);
+    } else {
+        const int passes = sort_passes(grid_.key_bits);
+        uint32_t* hash_out = (passes % 2) ? d_keys_b.get() : d_keys_a.get();
+        launch_hash(n_, d_pos.get(), grid_.dev, hash_out, d_keys_a.get(), clear, d_state.get(), stream_, &prof_);
+        launch_radix_sort(d_keys_a.get(), d_keys_b.get(), d_vals_a.get(), d_vals_b.get(), d_hist.get(), n_,
+                          grid_.key_bits, stream_, &prof_);
+        if (!(s_.flags & SPH_FULL_SORT)) launch_resort_samples(d_keys_a.get(), n_, resort_, stream_);
+    }
     fresh_ranges_ = false;
-    launch_radix_sort(d_keys_a.get(), d_keys_b.get(), d_vals_a.get(), d_vals_b.get(), d_hist.get(), n_,
-                      grid_.key_bits, stream_, &prof_);
+    sorted_hint_ = n_ > 0;
     launch_gather_fluid(n_, d_vals_a.get(), d_keys_a.get(), d_pos.get(), d_vel.get(),
                         s_.method == SPH_WCSPH ? d_vh.get() : nullptr, d_pos_s.get(), d_vel_s.get(),
                         s_.method == SPH_WCSPH ? d_vh_s.get() : nullptr, d_ranges.get(), grid_.dev, stream_, &prof_);
@@ -616,7 +629,12 @@ void SimulationBase::set_state(int which, const float* xyzw, int n) {
     std::vector<float4> v(n);
     for (int i = 0; i < n; ++i)
         v[i] = make_float4(xyzw[4 * i], xyzw[4 * i + 1], xyzw[4 * i + 2], which == 0 ? 1.0f : 0.0f);
-    if (which == 0) ensure_box(v);
+    if (which == 0) {
+        ensure_box(v);
+        // positions set from outside may lie anywhere: the next step sorts
+        // every key instead of placing the moved ones into the last order
+        sorted_hint_ = false;
+    }
     hip_check(hipMemcpyAsync(dst->get(), v.data(), n * sizeof(float4), hipMemcpyHostToDevice, stream_), "upload");
     synchronize();
 }

@@ -221,6 +221,8 @@ public:
     DeviceArray<uint32_t> d_keys_a, d_keys_b, d_hist;
     DeviceArray<int> d_vals_a, d_vals_b, d_nl, d_nc, d_bnl, d_bnc;
     DeviceArray<int2> d_ranges;
+    // work arrays of the sort that places only the keys that changed cell
+    DeviceArray<unsigned char> d_resort;
     DeviceArray<DevState> d_state;
     DeviceArray<unsigned long long> d_acc;
     float container[3] = {0, 0, 0};
@@ -245,6 +247,8 @@ protected:
     std::vector<std::shared_ptr<FluidVolume>> volumes_;
     int n_ = 0;
     bool fresh_ranges_ = true;  // the fluid range box holds no ranges yet
+    bool sorted_hint_ = false;  // d_keys_a holds a non-decreasing array of n_ keys (a sort's output)
+    ResortWork resort_;         // views into d_resort
     long long box_version_ = -1;  // grid box layout d_ranges was built for
     hipStream_t stream_ = nullptr;
     Profiler prof_;

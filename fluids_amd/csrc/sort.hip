@@ -109,4 +109,42 @@ void sort_pairs_noalloc(uint32_t* keys, int32_t* values, int n, int key_bits, vo
     sort_in_place(keys, values, n, key_bits, work, static_cast<hipStream_t>(stream));
 }
 
+// Non-zero when the hint decreases somewhere.
+__global__ __launch_bounds__(256) void k_check_non_decreasing(const uint32_t* __restrict__ a, int n,
+                                                              int* __restrict__ bad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i + 1 < n && a[i + 1] < a[i]) atomicOr(bad, 1);
+}
+
+void sort_near(const uint32_t* hint, const uint32_t* keys, uint32_t* sorted_keys, int32_t* perm, int n,
+               void* stream) {
+    if (n < 0) throw Error(SPH_ERR_INVALID, "sort: negative size");
+    if (n > MAX_SCENE_PARTICLES) throw Error(SPH_ERR_RANGE, "sort: at most 2^27 pairs");
+    if (n == 0) return;
+    if (!hint || !keys || !sorted_keys || !perm) throw Error(SPH_ERR_INVALID, "sort: NULL array");
+    if (sorted_keys == keys) throw Error(SPH_ERR_INVALID, "sph_sort_near: sorted_keys must not be keys");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int* bad = nullptr;
+    hip_check(hipMallocAsync(reinterpret_cast<void**>(&bad), 256 + resort_bytes(n), s), "hipMallocAsync");
+    struct Free {
+        void* p;
+        hipStream_t s;
+        ~Free() { (void)hipFreeAsync(p, s); }
+    } release{bad, s};
+    const ResortWork w = resort_carve(reinterpret_cast<unsigned char*>(bad) + 256, n);
+    hip_check(hipMemsetAsync(bad, 0, 4, s), "memset");
+    resort_clear(w, n, s);
+    hipLaunchKernelGGL(k_check_non_decreasing, dim3((n + 255) / 256), dim3(256), 0, s, hint, n, bad);
+    hip_check(hipGetLastError(), "check launch");
+    int h_bad = 0;
+    hip_check(hipMemcpyAsync(&h_bad, bad, 4, hipMemcpyDeviceToHost, s), "download");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    if (h_bad) throw Error(SPH_ERR_INVALID, "sph_sort_near: the hint is not non-decreasing");
+    if (sorted_keys != hint)
+        hip_check(hipMemcpyAsync(sorted_keys, hint, (size_t)n * 4, hipMemcpyDeviceToDevice, s), "copy");
+    launch_resort_samples(sorted_keys, n, w, s);
+    launch_resort(n, keys, sorted_keys, perm, w, s);
+    hip_check(hipGetLastError(), "sort launch");
+}
+
 }  // namespace sph

@@ -105,7 +105,7 @@ void validate(const sph_settings* s) {
     if (!(s->fluid_support_radius > 0.0f) || !(s->fluid_particle_radius > 0.0f))
         throw Error(SPH_ERR_INVALID, "radii must be positive");
     if (!(s->time_step > 0.0f)) throw Error(SPH_ERR_INVALID, "time_step must be positive");
-    if (s->flags & ~(SPH_FIX_WCSPH_DENSITY | SPH_FAST_MATH)) throw Error(SPH_ERR_INVALID, "unknown flag bits");
+    if (s->flags & ~(SPH_FIX_WCSPH_DENSITY | SPH_FAST_MATH | SPH_FULL_SORT)) throw Error(SPH_ERR_INVALID, "unknown flag bits");
 }
 
 // Batches and the adjoint run the IEEE kernels only (the adjoint's
@@ -985,6 +985,14 @@ extern "C" int sph_sort_pairs(uint32_t* keys, int32_t* values, int n, int key_bi
 extern "C" int sph_sort_pairs_noalloc(uint32_t* keys, int32_t* values, int n, int key_bits, void* stream) {
     return guard([&] {
         sort_pairs_noalloc(keys, values, n, key_bits, stream);
+        return SPH_OK;
+    });
+}
+
+extern "C" int sph_sort_near(const uint32_t* hint, const uint32_t* keys, uint32_t* sorted_keys, int32_t* perm,
+                             int n, void* stream) {
+    return guard([&] {
+        sort_near(hint, keys, sorted_keys, perm, n, stream);
         return SPH_OK;
     });
 }

@@ -172,10 +172,9 @@ def _counter_means(d, counter):
 def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, copy_bytes, tcc_dir=None):
     def per_kernel(d, counter):
         rows = [r for r in _counter_means(d, counter) if "sph::" in r["Kernel_Name"]]
-        rows = _full_grid(rows, lambda r: short_name(r["Kernel_Name"]))
         acc = collections.defaultdict(list)
-        for r in rows:
-            acc[short_name(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+        for k, r in _full_grid(rows, lambda r: short_name(r["Kernel_Name"])):
+            acc[k].append(float(r["Counter_Value"]))
         return {k: sum(v) / len(v) for k, v in acc.items()}, {k: len(v) for k, v in acc.items()}
 
     def copy_kib(d, counter):

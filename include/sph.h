@@ -51,6 +51,13 @@ extern "C" {
  * only. Batches (sph_batch_add_scene) and the adjoint reject it.
  */
 #define SPH_FAST_MATH 2
+/*
+ * Single-scene solvers: sort every step's cell keys with the full radix sort.
+ * By default a step sorts only the keys that changed since the previous
+ * step's sorted order and merges them in (same stable order, bit for bit);
+ * this flag is for comparing the two.
+ */
+#define SPH_FULL_SORT 4
 
 /*
  * PhysicsSettings (settings/physicssettings.h:5-25) and SimulationSettings
@@ -299,6 +306,21 @@ int sph_sort_pairs(uint32_t* keys, int32_t* values, int n, int key_bits, void* s
  * caller. */
 #define SPH_SORT_NOALLOC_MAX (1 << 20)
 int sph_sort_pairs_noalloc(uint32_t* keys, int32_t* values, int n, int key_bits, void* stream);
+
+/* The stable sort of (keys[i], i) for keys that differ little from a sorted
+ * array: the sort the single-scene solvers run every step, where particles
+ * are stored in the previous step's sorted order (replacing the per-step
+ * clsort of pcisphsimluation.cpp:86 / wcsphsimluation.cpp:54 by one
+ * that places only the keys that changed). hint: n non-decreasing keys (any
+ * such array gives the exact result; the closer to sorted(keys), the less
+ * work). keys: the n keys to sort. sorted_keys receives them in sorted order
+ * and perm the stable permutation (perm[r] = index of the r-th key), the same
+ * bits as sph_sort_pairs on (keys, 0 .. n-1). All device arrays; sorted_keys
+ * may be hint itself, not keys. Work arrays come from the stream-ordered
+ * allocator. Checks the hint (SPH_ERR_INVALID if it decreases) and therefore
+ * synchronises the stream. n up to 2^27. */
+int sph_sort_near(const uint32_t* hint, const uint32_t* keys, uint32_t* sorted_keys, int32_t* perm, int n,
+                  void* stream);
 
 /* -------------------------------------------------------------- samplers */
 /* Each returns the sample count; out (xyzw) receives up to cap samples. */
