@@ -43,6 +43,9 @@ WcAdjoint::WcAdjoint(const Settings& s, const float container[3], const float* p
     d_params.resize(3); d_capped.resize(1);
     hip_check(hipMemsetAsync(d_bnc.get(), 0, N * sizeof(int), stream_), "memset");
     hip_check(hipMemsetAsync(d_keys_a.get(), 0, N * sizeof(uint32_t), stream_), "memset");
+    d_resort.resize(resort_bytes(n_));
+    resort_ = resort_carve(d_resort.get(), n_);
+    resort_clear(resort_, n_, stream_);
     for (int k = 0; k < every_; ++k) tape_.emplace_back(new Slot());
     set_initial(pos, vel, vh);
 }
@@ -103,6 +106,7 @@ void WcAdjoint::upload_initial() {
     // a fresh range box: the first step has no previous keys to clear by
     hip_check(hipMemsetAsync(d_ranges.get(), 0, d_ranges.size() * sizeof(int2), stream_), "memset");
     fresh_ = true;
+    sorted_hint_ = false;
 }
 
 // One WCSPHSimulation::simulate (solver.cpp) writing its intermediate values
@@ -111,14 +115,23 @@ void WcAdjoint::run_step(int slot, const WcStepParams& W) {
     Slot& T = *tape_[slot];
     T.alloc(n_);
     hipStream_t s = stream_;
-    const int passes = sort_passes(grid_.key_bits);
-    uint32_t* hash_out = (passes % 2) ? d_keys_b.get() : d_keys_a.get();
     launch_step_begin(d_state.get(), s);
-    launch_hash(n_, d_pos.get(), grid_.dev, hash_out, d_keys_a.get(), fresh_ ? nullptr : d_ranges.get(),
-                d_state.get(), s, nullptr);
+    int2* clear = fresh_ ? nullptr : d_ranges.get();
+    if (sorted_hint_) {
+        // the solver's sort: particles are stored in the last step's order,
+        // so only the keys that changed cell are placed
+        launch_hash_resort(n_, d_pos.get(), grid_.dev, d_keys_b.get(), d_keys_a.get(), d_vals_a.get(), clear,
+                           d_state.get(), resort_, s, nullptr);
+    } else {
+        const int passes = sort_passes(grid_.key_bits);
+        uint32_t* hash_out = (passes % 2) ? d_keys_b.get() : d_keys_a.get();
+        launch_hash(n_, d_pos.get(), grid_.dev, hash_out, d_keys_a.get(), clear, d_state.get(), s, nullptr);
+        launch_radix_sort(d_keys_a.get(), d_keys_b.get(), d_vals_a.get(), d_vals_b.get(), d_hist.get(), n_,
+                          grid_.key_bits, s, nullptr);
+        launch_resort_samples(d_keys_a.get(), n_, resort_, s);
+    }
     fresh_ = false;
-    launch_radix_sort(d_keys_a.get(), d_keys_b.get(), d_vals_a.get(), d_vals_b.get(), d_hist.get(), n_,
-                      grid_.key_bits, s, nullptr);
+    sorted_hint_ = true;
     // the gather also records the permutation and moves the particle ids
     // into the new order (into d_ids_b, swapped in below)
     launch_gather_fluid_tape(n_, d_vals_a.get(), d_keys_a.get(), d_pos.get(), d_vel.get(), d_vh.get(), d_pos_s.get(),
@@ -155,6 +168,7 @@ void WcAdjoint::load_checkpoint(int k) {
     hip_check(hipMemcpyAsync(d_vel.get(), c.vel.get(), B, hipMemcpyDeviceToDevice, stream_), "copy");
     hip_check(hipMemcpyAsync(d_vh.get(), c.vh.get(), B, hipMemcpyDeviceToDevice, stream_), "copy");
     hip_check(hipMemcpyAsync(d_ids_a.get(), c.ids.get(), n_ * sizeof(int), hipMemcpyDeviceToDevice, stream_), "copy");
+    sorted_hint_ = false;  // the restored positions belong to an earlier order
 }
 
 void WcAdjoint::forward(int nsteps) {

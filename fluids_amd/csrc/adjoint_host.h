@@ -57,6 +57,7 @@ private:
     float container_[3] = {0, 0, 0};
     int n_ = 0, every_ = 20, steps_ = 0, taped_segment_ = -1;
     bool fresh_ = true, final_valid_ = false;
+    bool sorted_hint_ = false;  // d_keys_a holds the last step's sorted keys for the positions in d_pos
     float k_visc_ = 0, k_gas_ = 0, sigma_ = 0;
     double forward_ms_ = 0, backward_ms_ = 0;
     Grid grid_;
@@ -72,6 +73,8 @@ private:
     DeviceArray<int> d_vals_a, d_vals_b, d_bnc, d_ids_a, d_ids_b, d_capped;
     DeviceArray<int2> d_ranges;
     DeviceArray<DevState> d_state;
+    DeviceArray<unsigned char> d_resort;  // work arrays of the moved-keys sort
+    ResortWork resort_;
     DeviceArray<float4> d_g[2];
     DeviceArray<float4> d_w_abar, d_w_xbar, d_w_vbar, d_w_vhbar, d_w_nbar;
     DeviceArray<float> d_w_rbar, d_w_pbar, d_part_a, d_part_b;

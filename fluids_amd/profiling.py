@@ -124,13 +124,15 @@ def trace_summary(trace_dir, traffic_file=None):
     return "\n".join(out)
 
 
-def timeline_summary(trace_dir, marker="hash", skip=10):
+def timeline_summary(trace_dir, marker="gather_ranges", skip=10, exclude=("stats",)):
     """Step-by-step timeline of the solver's dispatches from a kernel trace:
-    a step starts at each full-grid dispatch of `marker` (the first kernel of
-    a step); the first `skip` steps (scene build, warm-up) are dropped. Per
-    step: span from the first dispatch's start to the next step's first
-    start, GPU busy time (the union of dispatch intervals), the idle time
-    between dispatches, and the dispatch count."""
+    a step starts at each full-grid dispatch of `marker` (one per step
+    whichever sort ran); the first `skip` steps (scene build, warm-up) and
+    the steps that hold a kernel named in `exclude` (the diagnostic counters
+    read between steps) are dropped. Per step: span from the first
+    dispatch's start to the next step's first start, GPU busy time (the
+    union of dispatch intervals), the idle time between dispatches, and the
+    dispatch count; means and medians over the steps."""
     rows = list(csv.DictReader(open(_csv(trace_dir, "kernel_trace.csv"))))
     rows = [r for r in rows if "sph::" in r["Kernel_Name"]]
     rows.sort(key=lambda r: int(r["Start_Timestamp"]))
@@ -139,6 +141,8 @@ def timeline_summary(trace_dir, marker="hash", skip=10):
     spans, busy, counts = [], [], []
     for a, b in zip(starts[skip:], starts[skip + 1:]):
         seg = rows[a:b]
+        if any(short_name(r["Kernel_Name"]) in exclude for r in seg):
+            continue
         t0, t1 = int(seg[0]["Start_Timestamp"]), int(rows[b]["Start_Timestamp"])
         iv = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"])) for r in seg)
         u, cs, ce = 0, iv[0][0], iv[0][1]
@@ -156,10 +160,13 @@ def timeline_summary(trace_dir, marker="hash", skip=10):
         return "no complete steps in the trace"
     n = len(spans)
     span, bsy = sum(spans) / n, sum(busy) / n
+    med = lambda v: sorted(v)[len(v) // 2]
     return "\n".join([
         "| steps | dispatches / step | span us / step | GPU busy us / step | idle between dispatches us / step | idle per dispatch us |",
         "|---|---|---|---|---|---|",
-        f"| {n} | {sum(counts) / n:.1f} | {span:.1f} | {bsy:.1f} | {span - bsy:.1f} | {(span - bsy) / (sum(counts) / n):.2f} |",
+        f"| {n} (mean) | {sum(counts) / n:.1f} | {span:.1f} | {bsy:.1f} | {span - bsy:.1f} | {(span - bsy) / (sum(counts) / n):.2f} |",
+        f"| {n} (median) | {med(counts)} | {med(spans):.1f} | {med(busy):.1f} | "
+        f"{med([a - b for a, b in zip(spans, busy)]):.1f} | - |",
     ])
 
 
