@@ -339,8 +339,11 @@ void launch_hash(int n, const float4* pos, const GridDev& g, uint32_t* keys, con
 //                    and of the moved count; the moved lanes of a wave with
 //                    one insertion point (a subgroup) reserve their slots
 //                    with one atomic and take them in (key, index) order
-//   k_resort_place   each tile's prefix from the tile sums before it; the
-//                    scans of D and of the moved flags inside the tile; the
+//   k_resort_tiles   (more than RESORT_SUM_TILES tiles) one workgroup: the
+//                    exclusive scans of the tile sums
+//   k_resort_place   each tile's prefix (from that scan, or the sum of the
+//                    tile sums before it); the scans of D and of the moved
+//                    flags inside the tile; the
 //                    unmoved elements to their ranks; each group's base rank
 //                    and size; the moved list
 //   k_resort_moved   groups that came from one subgroup straight to their
@@ -351,6 +354,9 @@ void launch_hash(int n, const float4* pos, const GridDev& g, uint32_t* keys, con
 // Counters return to zero by the end.
 constexpr int RESORT_MIN_TILE_SHIFT = 10;  // scan tiles of 1024 positions (256 threads x 4) ...
 constexpr int RESORT_MAX_TILES = 4096;     // ... or larger, so that there are at most this many
+// Up to this many tiles each k_resort_place workgroup sums the tile sums
+// before its own (no scan launch); beyond, k_resort_tiles scans them once.
+constexpr int RESORT_SUM_TILES = 1024;
 constexpr int RESORT_SMALL_GROUP = 64;     // groups ranked by their list up to this size
 constexpr int RESORT_LDS_GROUP = 4096;     // larger groups sort in LDS up to this size, else in place
 constexpr int RESORT_LIST_BLOCKS = 512;    // workgroups that walk the list of moved elements
@@ -384,12 +390,13 @@ ResortSizes resort_sizes(int n) {
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // Byte sizes of the work arrays in the order resort_carve lays them out.
-constexpr int RESORT_PARTS = 14;
+constexpr int RESORT_PARTS = 16;
 static void resort_parts(int n, size_t out[RESORT_PARTS]) {
     const ResortSizes z = resort_sizes(n);
-    const size_t sizes[RESORT_PARTS] = {z.mask_words * 8, z.padded * 4, z.padded * 8, z.tiles * 4 * RESORT_PAD, z.tiles * 4 * RESORT_PAD,
-                                        z.elems * 16,     z.elems * 4,  z.elems * 4,  z.elems * 4, z.big * 4,
-                                        4,                4,            z.s1 * 4,     z.s2 * 4};
+    const size_t sizes[RESORT_PARTS] = {z.mask_words * 8, z.padded * 4,  z.padded * 8, z.tiles * 4 * RESORT_PAD,
+                                        z.tiles * 4 * RESORT_PAD, z.elems * 16, z.elems * 4, z.elems * 4,
+                                        z.elems * 4,      z.big * 4,     4,            4,
+                                        z.s1 * 4,         z.s2 * 4,      z.tiles * 4,  z.tiles * 4};
     for (int k = 0; k < RESORT_PARTS; ++k) out[k] = sizes[k];
 }
 
@@ -424,6 +431,8 @@ ResortWork resort_carve(void* block, int n) {
     w.nmoved = static_cast<int*>(a[11]);
     w.S = static_cast<uint32_t*>(a[12]);
     w.S2 = static_cast<uint32_t*>(a[13]);
+    w.Tx = static_cast<int*>(a[14]);
+    w.TFx = static_cast<int*>(a[15]);
     return w;
 }
 
@@ -598,6 +607,22 @@ __global__ __launch_bounds__(256) void k_resort_flag(int n, int tile_shift, cons
     }
 }
 
+// Inclusive scan of one value per thread over a 1024-thread workgroup
+// (wsum: 16 words of LDS).
+__device__ __forceinline__ int block_incl_scan_1024(int v, int* wsum) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(v, off, 64);
+        if (lane >= off) v += u;
+    }
+    if (lane == 63) wsum[wave] = v;
+    __syncthreads();
+    int add = 0;
+    for (int q = 0; q < wave; ++q) add += wsum[q];
+    __syncthreads();
+    return v + add;
+}
+
 // Sums of the padded counters a[t * RESORT_PAD] and b[t * RESORT_PAD] over
 // t < m (every thread of the workgroup gets them).
 __device__ __forceinline__ int2 block_sum2(const int* __restrict__ a, const int* __restrict__ b, int m,
@@ -622,19 +647,56 @@ __device__ __forceinline__ int2 block_sum2(const int* __restrict__ a, const int*
     return r;
 }
 
+// Exclusive scans of the padded tile sums (D into Tx, the moved count into
+// TFx) and the total moved count, in one workgroup; leaves the sums zero for
+// the next sort and clears the count of large groups.
+__global__ __launch_bounds__(1024) void k_resort_tiles(int ntiles, ResortWork w) {
+    __shared__ int wsum[16];
+    __shared__ int tot[2];
+    const int t = threadIdx.x;
+    int carry = 0, carry_f = 0;
+    for (int base = 0; base < ntiles; base += 1024) {
+        const int idx = base + t;
+        const int v = idx < ntiles ? w.T[idx * RESORT_PAD] : 0;
+        const int f = idx < ntiles ? w.TF[idx * RESORT_PAD] : 0;
+        const int sv = block_incl_scan_1024(v, wsum);
+        const int sf = block_incl_scan_1024(f, wsum);
+        if (idx < ntiles) {
+            w.Tx[idx] = carry + sv - v;
+            w.TFx[idx] = carry_f + sf - f;
+            w.T[idx * RESORT_PAD] = 0;
+            w.TF[idx * RESORT_PAD] = 0;
+        }
+        if (t == 1023) {
+            tot[0] = sv;
+            tot[1] = sf;
+        }
+        __syncthreads();
+        carry += tot[0];
+        carry_f += tot[1];
+        __syncthreads();
+    }
+    if (t == 0) {
+        *w.nbig = 0;
+        *w.nmoved = carry_f;
+    }
+}
+
 // One tile of 2^tile_shift positions, `per` consecutive per thread (a
 // multiple of 4): the unmoved elements written to their ranks, the moved
 // ones to the moved list (index order), and for each group its base rank
 // p + X[p], size and list head (grp[p]), with H and the list head reset.
 // H and the list heads are padded to whole tiles.
-__global__ __launch_bounds__(256) void k_resort_place(int n, int tile_shift, int ntiles,
+__global__ __launch_bounds__(256) void k_resort_place(int n, int tile_shift, int ntiles, int scanned,
                                                       const uint32_t* __restrict__ keys, ResortWork w,
                                                       uint32_t* __restrict__ sk, int* __restrict__ perm) {
     __shared__ uint32_t wtot[4];
     __shared__ int2 part[4];
     const int per = (1 << tile_shift) >> 8;
     const int base = (blockIdx.x << tile_shift) + threadIdx.x * per;
-    const int2 before = block_sum2(w.T, w.TF, blockIdx.x, part);
+    // the tile sums before this tile: scanned by k_resort_tiles, or summed here
+    const int2 before = scanned ? make_int2(w.Tx[blockIdx.x], w.TFx[blockIdx.x])
+                                : block_sum2(w.T, w.TF, blockIdx.x, part);
     int s = 0, nf = 0;
     for (int c = 0; c < per; c += 4) {
         const int4 hv = *reinterpret_cast<const int4*>(w.H + base + c);
@@ -646,8 +708,10 @@ __global__ __launch_bounds__(256) void k_resort_place(int n, int tile_shift, int
     const int incl_f = (int)block_incl_scan_256((uint32_t)nf, wtot);
     int x = before.x + incl - s;
     int f = before.y + incl_f - nf;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *w.nbig = 0;
-    if (blockIdx.x == ntiles - 1 && threadIdx.x == 255) *w.nmoved = before.y + incl_f;
+    if (!scanned) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) *w.nbig = 0;
+        if (blockIdx.x == ntiles - 1 && threadIdx.x == 255) *w.nmoved = before.y + incl_f;
+    }
     for (int c = 0; c < per; c += 4) {
         const int4 hv = *reinterpret_cast<const int4*>(w.H + base + c);
         const int h[4] = {hv.x, hv.y, hv.z, hv.w};
@@ -749,15 +813,17 @@ __device__ __forceinline__ bool ranked_group(int4 rc, int2 e) { return rc.z == e
 // gathered at base + slot for k_resort_groups (and a group of more than
 // RESORT_SMALL_GROUP is listed for a workgroup sort). With a range box, each
 // moved element clears the range of the cell it left (cells that keep
-// particles are rewritten by the gather that follows). Also returns the tile
-// sums to zero.
-__global__ __launch_bounds__(256) void k_resort_moved(int n, int ntiles, int big_cap, const uint32_t* __restrict__ keys,
-                                                      ResortWork w, uint32_t* __restrict__ sk,
-                                                      int* __restrict__ perm, int2* __restrict__ R, GridDev g) {
+// particles are rewritten by the gather that follows).
+__global__ __launch_bounds__(256) void k_resort_moved(int n, int ntiles, int scanned, int big_cap,
+                                                      const uint32_t* __restrict__ keys, ResortWork w,
+                                                      uint32_t* __restrict__ sk, int* __restrict__ perm,
+                                                      int2* __restrict__ R, GridDev g) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
-    for (int t = gid; t < ntiles; t += stride) {
-        w.T[t * RESORT_PAD] = 0;
-        w.TF[t * RESORT_PAD] = 0;
+    if (!scanned) {  // the tile sums back to zero (k_resort_tiles clears them when it runs)
+        for (int t = gid; t < ntiles; t += stride) {
+            w.T[t * RESORT_PAD] = 0;
+            w.TF[t * RESORT_PAD] = 0;
+        }
     }
     const int nm = *w.nmoved;
     for (int q = gid; q < nm; q += stride) {
@@ -824,12 +890,20 @@ static void resort_rest(int n, const uint32_t* keys, uint32_t* keys_out, int* pe
                         const GridDev& g, hipStream_t s, Profiler* prof) {
     const int sh = resort_tile_shift(n), ntiles = resort_tiles(n);
     const int list_blocks = std::min(RESORT_LIST_BLOCKS, (n + 255) / 256);
+    const int scanned = ntiles > RESORT_SUM_TILES ? 1 : 0;
+    if (scanned) {
+        PROF_BEGIN(prof, "resort_scan", s);
+        hipLaunchKernelGGL(k_resort_tiles, dim3(1), dim3(1024), 0, s, ntiles, w);
+        PROF_END(prof, s);
+    }
     PROF_BEGIN(prof, "resort_place", s);
-    hipLaunchKernelGGL(k_resort_place, dim3(ntiles), dim3(256), 0, s, n, sh, ntiles, keys, w, keys_out, perm);
+    hipLaunchKernelGGL(k_resort_place, dim3(ntiles), dim3(256), 0, s, n, sh, ntiles, scanned, keys, w, keys_out,
+                       perm);
     PROF_END(prof, s);
     const int big_cap = (int)resort_sizes(n).big;
     PROF_BEGIN(prof, "resort_moved", s);
-    hipLaunchKernelGGL(k_resort_moved, dim3(list_blocks), dim3(256), 0, s, n, ntiles, big_cap, keys, w, keys_out, perm,
+    hipLaunchKernelGGL(k_resort_moved, dim3(list_blocks), dim3(256), 0, s, n, ntiles, scanned, big_cap, keys, w,
+                       keys_out, perm,
                        R, g);
     PROF_END(prof, s);
     PROF_BEGIN(prof, "resort_groups", s);

@@ -109,6 +109,8 @@ struct ResortWork {
     int2* grp;                  // padded: (base rank, size) of each insertion point's group
     int* T;                     // tiles x 32 (one counter per 128-byte line): per-tile sums of H - moved
     int* TF;                    // tiles x 32: per-tile moved counts
+    int* Tx;                    // tiles: exclusive scan of the sums of H - moved
+    int* TFx;                   // tiles: exclusive scan of the moved counts
     int4* rec;                  // elems: per moved element (insertion point, slot, size of its wave's
                                 //        subgroup if that took the group's first slots else 0, key in the hint)
     int* mlist;                 // elems: the moved elements in index order

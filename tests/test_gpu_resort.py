@@ -5,8 +5,9 @@ permutation equals order = argsort(keys, kind="stable"), bit for bit. The
 cases cover the kernels' paths: elements that keep their key, moved elements
 alone at their insertion point, groups of 2-64 (ranked per element), of
 65-4096 (sorted in LDS) and beyond (sorted in place), ties between moved and
-unmoved equal keys, hints unrelated to the keys, and the sizes around the
-64-element mask words and 4096-position scan tiles. The solver test runs a
+unmoved equal keys, hints unrelated to the keys, the sizes around the
+64-element mask words and 1024-position scan tiles, and a size past 1024
+tiles (the tile sums then go through the separate scan launch). The solver test runs a
 scene with SPH_FULL_SORT (the radix sort of every key each step) beside the
 default and compares every field after each step."""
 import ctypes as C
@@ -55,7 +56,7 @@ def near_sorted(rng, n, frac, span, bits=27):
     return hint.astype(np.uint32), keys.astype(np.uint32)
 
 
-@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 4095, 4096, 4097, 8191, 300_000, 1_000_000])
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 4095, 4096, 4097, 8191, 300_000, 1_000_000, 1_500_000])
 def test_moved_fraction_with_small_moves(n):
     print("case near-sorted", n, flush=True)
     rng = np.random.default_rng(n)
