@@ -98,6 +98,10 @@ void BatchWCSPH::finalize() {
     d_ranges.resize((size_t)std::max(roff, 1LL));
     d_scenes.resize(scenes_.size());
     d_state.resize(1);
+    d_resort.resize(resort_bytes(n_));
+    resort_ = resort_carve(d_resort.get(), n_);
+    resort_clear(resort_, n_, stream_);
+    sorted_hint_ = false;
     hip_check(hipMemsetAsync(d_ranges.get(), 0, d_ranges.size() * sizeof(int2), stream_), "memset");
     hip_check(hipMemsetAsync(d_bnc.get(), 0, N * sizeof(int), stream_), "memset");
     hip_check(hipMemsetAsync(d_keys_a.get(), 0, N * sizeof(uint32_t), stream_), "memset");
@@ -124,13 +128,20 @@ void BatchWCSPH::require_final() const {
 void BatchWCSPH::enqueue_step(bool clear) {
     hipStream_t s = stream_;
     Profiler* prof = &prof_;
-    const int passes = sort_passes(key_bits_, sid_bits_);
-    uint32_t* hash_out = (passes % 2) ? d_keys_b.get() : d_keys_a.get();
     launch_step_begin(d_state.get(), s);
-    launch_batch_hash(n_, d_sid.get(), d_scenes.get(), d_pos.get(), hash_out, d_keys_a.get(), d_ranges.get(), clear,
-                      d_state.get(), cfg_[K_HASH], s, prof);
-    launch_radix_sort(d_keys_a.get(), d_keys_b.get(), d_vals_a.get(), d_vals_b.get(), d_hist.get(), n_, key_bits_, s,
-                      prof, d_sid.get(), sid_bits_);
+    if (sorted_hint_) {
+        // every scene keeps its segment and its particles are stored in the
+        // last step's order: only the keys that changed cell are placed
+        launch_batch_hash_resort(n_, d_sid.get(), d_scenes.get(), d_pos.get(), d_keys_b.get(), d_keys_a.get(),
+                                 d_vals_a.get(), clear ? d_ranges.get() : nullptr, d_state.get(), resort_, s, prof);
+    } else {
+        const int passes = sort_passes(key_bits_, sid_bits_);
+        uint32_t* hash_out = (passes % 2) ? d_keys_b.get() : d_keys_a.get();
+        launch_batch_hash(n_, d_sid.get(), d_scenes.get(), d_pos.get(), hash_out, d_keys_a.get(), d_ranges.get(),
+                          clear, d_state.get(), cfg_[K_HASH], s, prof);
+        launch_radix_sort(d_keys_a.get(), d_keys_b.get(), d_vals_a.get(), d_vals_b.get(), d_hist.get(), n_, key_bits_,
+                          s, prof, d_sid.get(), sid_bits_);
+    }
     launch_batch_gather(n_, d_sid.get(), d_scenes.get(), d_vals_a.get(), d_keys_a.get(), d_pos.get(), d_vel.get(),
                         d_vh.get(), d_pos_s.get(), d_vel_s.get(), d_vh_s.get(), d_ranges.get(), cfg_[K_GATHER], s,
                         prof);
@@ -158,9 +169,10 @@ void BatchWCSPH::step(int nsteps) {
         if (n_ == 0) return;
         // the first step has no previous keys to clear ranges by; profiling
         // records events around each launch, which a graph cannot hold
-        if (fresh_ || !use_graph_ || prof_.enabled) {
+        if (fresh_ || !sorted_hint_ || !use_graph_ || prof_.enabled) {
             enqueue_step(!fresh_);
             fresh_ = false;
+            sorted_hint_ = true;
             continue;
         }
         if (!exec_) {
@@ -242,6 +254,7 @@ void BatchWCSPH::set_scene_state(int which, int scene, const float* in, int n) {
     float4* dst = which == 0 ? d_pos.get() : which == 1 ? d_vel.get() : d_vh.get();
     hip_check(hipMemcpyAsync(dst + S.lo, v.data(), n * sizeof(float4), hipMemcpyHostToDevice, stream_), "upload");
     synchronize();
+    if (which == 0) sorted_hint_ = false;  // new positions: the next step sorts every key
 }
 
 long long BatchWCSPH::range_cells() const { return (long long)d_ranges.size(); }
@@ -352,6 +365,7 @@ int BatchWCSPH::autotune(const char* cache_path, int trials) {
     hip_check(hipMemcpyAsync(d_vel.get(), save_v.get(), n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_), "copy");
     hip_check(hipMemcpyAsync(d_vh.get(), save_h.get(), n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_), "copy");
     synchronize();
+    sorted_hint_ = false;  // the restored positions belong to an earlier order
     if (cache_path && *cache_path) save_tuning(cache_path);
     return 0;
 }

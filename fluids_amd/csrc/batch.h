@@ -39,6 +39,12 @@ struct LaunchCfg {
 void launch_batch_hash(int n, const int* sid, const BatchScene* sc, const float4* pos, uint32_t* keys,
                        const uint32_t* prev, int2* R, bool clear, DevState* st, const LaunchCfg& c, hipStream_t s,
                        Profiler* prof);
+// The batch's keys sorted by (scene, key) with `sorted` (the last step's
+// sorted keys; every scene keeps its segment) as the hint, placing only the
+// keys that changed cell; the batched form of launch_hash_resort.
+void launch_batch_hash_resort(int n, const int* sid, const BatchScene* sc, const float4* pos, uint32_t* keys,
+                              uint32_t* sorted, int* perm, int2* R, DevState* st, const ResortWork& w, hipStream_t s,
+                              Profiler* prof);
 void launch_batch_gather(int n, const int* sid, const BatchScene* sc, const int* perm, const uint32_t* keys_s,
                          const float4* pos, const float4* vel, const float4* vh, float4* pos_s, float4* vel_s,
                          float4* vh_s, int2* R, const LaunchCfg& c, hipStream_t s, Profiler* prof);

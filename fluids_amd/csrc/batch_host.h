@@ -66,6 +66,9 @@ private:
     std::vector<BatchScene> scenes_;
     int n_ = 0, key_bits_ = 1, sid_bits_ = 8;
     bool finalized_ = false, fresh_ = true, use_graph_ = true;
+    // d_keys_a holds the last step's sorted keys for the positions in d_pos:
+    // the next step places only the keys that changed cell
+    bool sorted_hint_ = false;
     LaunchCfg cfg_[K_COUNT];
     double tuned_ms_[K_COUNT] = {0, 0, 0, 0, 0};
     hipStream_t stream_ = nullptr;
@@ -80,6 +83,8 @@ private:
     DeviceArray<int2> d_ranges;
     DeviceArray<BatchScene> d_scenes;
     DeviceArray<DevState> d_state;
+    DeviceArray<unsigned char> d_resort;  // work arrays of the moved-keys sort
+    ResortWork resort_;
 };
 
 }  // namespace sph

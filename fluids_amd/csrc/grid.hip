@@ -498,18 +498,61 @@ __device__ __forceinline__ int2 key_range(uint32_t k, const int2* __restrict__ R
     return b >= 0 ? R[b] : make_int2(0, 0);
 }
 
-// Insertion point of (k, m): from P's range of k when P holds k (the range
-// is accepted only if P confirms both ends), else from the end of the
-// nearest occupied predecessor's range when P confirms that k falls right
-// there, else by search. The confirming reads are issued together (indices
-// clamped into P; the tests only use the ones that apply).
-__device__ int insertion_point(const uint32_t* __restrict__ P, const ResortWork& w, int n, uint32_t k, int m,
-                               const int2* __restrict__ R, const GridDev& g) {
+// Insertion point of (k, m) inside a segment [lo, hi) of P by search:
+// galloping outward from m (a particle that changed cell usually keeps a
+// nearby rank), then binary inside the bracket. Invariant: the pairs below
+// lo are below (k, m), the pair at hi (or the segment's end) is not.
+__device__ int segment_search(const uint32_t* __restrict__ P, int lo, int hi, uint32_t k, int m) {
+    const int end = hi;
+    if (k > P[m]) {
+        lo = m + 1;
+        for (long long b = 1;; b <<= 1) {
+            const long long x = (long long)m + b;
+            if (x >= end) break;
+            if (below(P[x], (int)x, k, m)) {
+                lo = (int)x + 1;
+            } else {
+                hi = (int)x;
+                break;
+            }
+        }
+    } else {
+        const int start = lo;
+        hi = m;
+        for (long long b = 1;; b <<= 1) {
+            const long long x = (long long)m - b;
+            if (x < start) break;
+            if (below(P[x], (int)x, k, m)) {
+                lo = (int)x + 1;
+                break;
+            }
+            hi = (int)x;
+        }
+    }
+    while (lo < hi) {
+        const int mid = lo + ((hi - lo) >> 1);
+        if (below(P[mid], mid, k, m)) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Insertion point of (k, m) inside the segment [lo, hi) of P that holds m
+// (the whole array for one scene, the scene's particles in a batch): from
+// P's range of k when P holds k (the range is accepted only if P confirms
+// both ends), else from the end of the nearest occupied predecessor's range
+// when P confirms that k falls right there, else by search (through the
+// samples for a whole array, galloping inside a segment). The confirming
+// reads are issued together (indices clamped into the segment; the tests
+// only use the ones that apply).
+__device__ int insertion_point(const uint32_t* __restrict__ P, const ResortWork& w, int n, int lo, int hi, uint32_t k,
+                               int m, const int2* __restrict__ R, const GridDev& g) {
     if (R) {
         const int2 r = key_range(k, R, g);
-        if (r.y > r.x && r.x >= 0 && r.y <= n) {
-            const uint32_t a0 = P[r.x > 0 ? r.x - 1 : 0], a1 = P[r.x], a2 = P[r.y - 1], a3 = P[r.y < n ? r.y : n - 1];
-            if ((a1 == k) & (a2 == k) & ((r.x == 0) | (a0 < k)) & ((r.y == n) | (a3 > k)))
+        if (r.y > r.x && r.x >= lo && r.y <= hi) {
+            const uint32_t a0 = P[r.x > lo ? r.x - 1 : lo], a1 = P[r.x], a2 = P[r.y - 1];
+            const uint32_t a3 = P[r.y < hi ? r.y : hi - 1];
+            if ((a1 == k) & (a2 == k) & ((r.x == lo) | (a0 < k)) & ((r.y == hi) | (a3 > k)))
                 return m < r.x ? r.x : (m > r.y ? r.y : m);
         } else if (r.y <= r.x) {
             int2 q[RESORT_PRED];
@@ -520,26 +563,39 @@ __device__ int insertion_point(const uint32_t* __restrict__ P, const ResortWork&
 #pragma unroll
             for (int t = RESORT_PRED - 1; t >= 0; --t)
                 if (q[t].y > q[t].x) p = q[t].y;  // the nearest occupied predecessor wins
-            if (p > 0 && p <= n) {
-                const uint32_t a0 = P[p - 1], a1 = P[p < n ? p : n - 1];
-                if ((a0 < k) & ((p == n) | (a1 > k))) return p;
+            if (p > lo && p <= hi) {
+                const uint32_t a0 = P[p - 1], a1 = P[p < hi ? p : hi - 1];
+                if ((a0 < k) & ((p == hi) | (a1 > k))) return p;
             }
         }
     }
-    return insertion_search(P, w, n, k, m);
+    return (lo == 0 && hi == n) ? insertion_search(P, w, n, k, m) : segment_search(P, lo, hi, k, m);
 }
 
 // HASH: the keys are the cells of pos (compute_hashes, with k_hash's
 // counters) and are written to keys; otherwise keys holds them. R (may be
-// null): the range box of P's cells.
+// null): the range box of P's cells. With sid and sc (a batch), particle i
+// belongs to scene sc[sid[i]]: its grid, its range box (R + roff) and its
+// segment [lo, hi), inside which it stays.
 template <bool HASH>
-__global__ __launch_bounds__(256) void k_resort_flag(int n, int tile_shift, const float4* __restrict__ pos, GridDev g,
+__global__ __launch_bounds__(256) void k_resort_flag(int n, int tile_shift, const float4* __restrict__ pos, GridDev g0,
                                                      uint32_t* __restrict__ keys, const uint32_t* __restrict__ P,
-                                                     const int2* __restrict__ R, DevState* st, ResortWork w) {
+                                                     const int2* __restrict__ R0, DevState* st, ResortWork w,
+                                                     const int* __restrict__ sid, const BatchScene* __restrict__ sc) {
     const int i = (HASH ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x) * blockDim.x + threadIdx.x;
     bool moved = false;
     uint32_t k = 0, pk = 0;
+    GridDev g = g0;
+    const int2* R = R0;
+    int lo = 0, hi = n;
     if (i < n) {
+        if (sid) {
+            const BatchScene& S = sc[sid[i]];
+            g = S.g;
+            R = R0 ? R0 + S.roff : nullptr;
+            lo = S.lo;
+            hi = S.hi;
+        }
         pk = P[i];
         if (HASH) {
             k = cell_key(pos[i], g, st);
@@ -554,7 +610,7 @@ __global__ __launch_bounds__(256) void k_resort_flag(int n, int tile_shift, cons
     const int i0 = i - lane;  // blocks are whole waves, so i0 is 64-aligned
     if (lane == 0 && i0 < n) w.moved[i0 >> 6] = b;
     if (!b) return;
-    const int p = moved ? insertion_point(P, w, n, k, i, R, g) : -1;
+    const int p = moved ? insertion_point(P, w, n, lo, hi, k, i, R, g) : -1;
     // The wave's moved lanes with the same insertion point form a subgroup:
     // its first lane (the leader) reserves the subgroup's slots with one
     // atomic, and each member's slot is the leader's base plus the member's
@@ -817,7 +873,8 @@ __device__ __forceinline__ bool ranked_group(int4 rc, int2 e) { return rc.z == e
 __global__ __launch_bounds__(256) void k_resort_moved(int n, int ntiles, int scanned, int big_cap,
                                                       const uint32_t* __restrict__ keys, ResortWork w,
                                                       uint32_t* __restrict__ sk, int* __restrict__ perm,
-                                                      int2* __restrict__ R, GridDev g) {
+                                                      int2* __restrict__ R, GridDev g,
+                                                      const int* __restrict__ sid, const BatchScene* __restrict__ sc) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
     if (!scanned) {  // the tile sums back to zero (k_resort_tiles clears them when it runs)
         for (int t = gid; t < ntiles; t += stride) {
@@ -830,7 +887,14 @@ __global__ __launch_bounds__(256) void k_resort_moved(int n, int ntiles, int sca
         const int i = w.mlist[q];
         if ((unsigned)i >= (unsigned)n) continue;  // never, for consistent work arrays
         const int4 rc = w.rec[i];  // (insertion point, slot, first subgroup's size or 0, key in the hint)
-        if (R) clear_range((uint32_t)rc.w, g, R);
+        if (R) {
+            if (sid) {
+                const BatchScene& S = sc[sid[i]];
+                clear_range((uint32_t)rc.w, S.g, R + S.roff);
+            } else {
+                clear_range((uint32_t)rc.w, g, R);
+            }
+        }
         const int2 e = w.grp[rc.x];
         const int b = e.x, gs = e.y;
         if (gs < 1 || b < 0 || b > n - gs || (unsigned)rc.y >= (unsigned)gs) continue;
@@ -887,7 +951,7 @@ __global__ __launch_bounds__(256) void k_resort_groups(int n, int list_blocks, i
 // keys_out/perm: the sorted keys and the permutation. P (read) and keys_out
 // may be the same array: P is last read by k_resort_flag.
 static void resort_rest(int n, const uint32_t* keys, uint32_t* keys_out, int* perm, const ResortWork& w, int2* R,
-                        const GridDev& g, hipStream_t s, Profiler* prof) {
+                        const GridDev& g, const int* sid, const BatchScene* sc, hipStream_t s, Profiler* prof) {
     const int sh = resort_tile_shift(n), ntiles = resort_tiles(n);
     const int list_blocks = std::min(RESORT_LIST_BLOCKS, (n + 255) / 256);
     const int scanned = ntiles > RESORT_SUM_TILES ? 1 : 0;
@@ -903,8 +967,7 @@ static void resort_rest(int n, const uint32_t* keys, uint32_t* keys_out, int* pe
     const int big_cap = (int)resort_sizes(n).big;
     PROF_BEGIN(prof, "resort_moved", s);
     hipLaunchKernelGGL(k_resort_moved, dim3(list_blocks), dim3(256), 0, s, n, ntiles, scanned, big_cap, keys, w,
-                       keys_out, perm,
-                       R, g);
+                       keys_out, perm, R, g, sid, sc);
     PROF_END(prof, s);
     PROF_BEGIN(prof, "resort_groups", s);
     hipLaunchKernelGGL(k_resort_groups, dim3(list_blocks + RESORT_BIG_BLOCKS), dim3(256), 0, s, n, list_blocks,
@@ -917,16 +980,27 @@ void launch_hash_resort(int n, const float4* pos, const GridDev& g, uint32_t* ke
     if (n <= 0) return;
     PROF_BEGIN(prof, "hash_moved", s);
     hipLaunchKernelGGL(k_resort_flag<true>, dim3((n + 255) / 256), dim3(256), 0, s, n, resort_tile_shift(n), pos, g,
-                       keys, sorted, R, st, w);
+                       keys, sorted, R, st, w, nullptr, nullptr);
     PROF_END(prof, s);
-    resort_rest(n, keys, sorted, perm, w, R, g, s, prof);
+    resort_rest(n, keys, sorted, perm, w, R, g, nullptr, nullptr, s, prof);
+}
+
+void launch_batch_hash_resort(int n, const int* sid, const BatchScene* sc, const float4* pos, uint32_t* keys,
+                              uint32_t* sorted, int* perm, int2* R, DevState* st, const ResortWork& w, hipStream_t s,
+                              Profiler* prof) {
+    if (n <= 0) return;
+    PROF_BEGIN(prof, "batch_hash_moved", s);
+    hipLaunchKernelGGL(k_resort_flag<true>, dim3((n + 255) / 256), dim3(256), 0, s, n, resort_tile_shift(n), pos,
+                       GridDev{}, keys, sorted, R, st, w, sid, sc);
+    PROF_END(prof, s);
+    resort_rest(n, keys, sorted, perm, w, R, GridDev{}, sid, sc, s, prof);
 }
 
 void launch_resort(int n, const uint32_t* keys, uint32_t* sorted, int* perm, const ResortWork& w, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_resort_flag<false>, dim3((n + 255) / 256), dim3(256), 0, s, n, resort_tile_shift(n), nullptr,
-                       GridDev{}, const_cast<uint32_t*>(keys), sorted, nullptr, nullptr, w);
-    resort_rest(n, keys, sorted, perm, w, nullptr, GridDev{}, s, nullptr);
+                       GridDev{}, const_cast<uint32_t*>(keys), sorted, nullptr, nullptr, w, nullptr, nullptr);
+    resort_rest(n, keys, sorted, perm, w, nullptr, GridDev{}, nullptr, nullptr, s, nullptr);
 }
 
 // ------------------------------------------------------ gather + ranges
