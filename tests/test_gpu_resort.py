@@ -167,3 +167,37 @@ def test_solver_sort_paths_give_identical_steps(method):
         for f in fields:
             a, b = sims[0].field(f), sims[1].field(f)
             assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), (step, f)
+
+
+def test_solver_sort_paths_agree_when_many_positions_turn_non_finite():
+    """Failure path forced from the host: 40 % of a scene's velocities are
+    set to NaN, so after one step those particles (and, through viscosity,
+    their neighbours) have non-finite positions, which all hash to one cell:
+    the next step's sort sees one group of several hundred moved particles
+    (sorted by a workgroup). The moved-keys path and the full radix sort give
+    the same bits in every field for three steps."""
+    import dataclasses
+    from fluids_amd import scenes
+    base = scenes.small_pcisph(seed=5)
+    print("case nan group", base.n_fluid, flush=True)
+    sims = []
+    for flags in (0, SPH_FULL_SORT):
+        st = dict(base.settings)
+        st["flags"] = st.get("flags", 0) | flags
+        sims.append(scenes.build(dataclasses.replace(base, settings=st)))
+    rng = np.random.default_rng(1)
+    vel = sims[0].velocities()
+    bad = rng.random(vel.shape[0]) < 0.4
+    vel[bad, :3] = np.nan
+    for s in sims:
+        s.set_velocities(vel)
+    fields = ["keys", "perm", "ncount", "nlist", "pos", "vel", "density", "pressure"]
+    for step in range(3):
+        for s in sims:
+            s.step()
+        for f in fields:
+            a, b = sims[0].field(f), sims[1].field(f)
+            assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), (step, f)
+    nan = int((~np.isfinite(sims[0].field("pos")[:, :3]).all(1)).sum())
+    assert nan >= int(bad.sum())
+    assert sims[0].step_stats()["nan_positions"] == sims[1].step_stats()["nan_positions"] == nan
