@@ -12,6 +12,8 @@ Two summaries are produced from rocprofv3's CSV output:
 * ``pmc``: per-kernel HBM traffic from two ``--pmc`` runs (FETCH_SIZE, then
   WRITE_SIZE, in KiB per dispatch), each scaled by what the same counter reads
   for a device copy of known size.
+* ``units``: per-kernel busy fractions of the vector-memory path (TA, TD, L1
+  tag lookups) and VALU issue rates from three more ``--pmc`` runs.
 
 Usage::
 
@@ -21,6 +23,7 @@ Usage::
         --copy-bytes 268435456 [--tcc-dir <hit_miss_dir>] > traffic.md
     python -m fluids_amd.profiling stats <trace_dir or *kernel_stats.csv> > stats.md
     python -m fluids_amd.profiling timeline <trace_dir> [--skip 10] > timeline.md
+    python -m fluids_amd.profiling units <ta_td_dir> <sq_dir> <tcp_dir> > units.md
 
 ``bench.py`` reads the machine-readable ``traffic`` rows of a ``pmc`` summary
 (``read_traffic``) to fill its roofline's ``traffic`` field.
@@ -216,6 +219,84 @@ def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, copy_bytes
     return "\n".join(out)
 
 
+# Counter passes of `units`: each list fits one rocprofv3 --pmc run (at most
+# 8 SQ, 4 TCP, 2 TA, 2 TD, 1 GRBM hardware counters per run).
+UNIT_PASSES = {
+    "ta_td": ["GRBM_GUI_ACTIVE", "TA_BUSY_avr", "TA_BUSY_max", "TA_ADDR_STALLED_BY_TC_CYCLES_sum",
+              "TD_TD_BUSY_sum", "TD_TC_STALL_sum"],
+    "sq": ["GRBM_GUI_ACTIVE", "SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_VMEM_RD", "SQ_INSTS_VMEM_WR",
+           "SQ_ACTIVE_INST_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS", "SQ_BUSY_CU_CYCLES"],
+    "tcp": ["GRBM_GUI_ACTIVE", "TCP_TOTAL_CACHE_ACCESSES_sum", "TCP_TCC_READ_REQ_sum",
+            "TCP_PENDING_STALL_CYCLES_sum", "TCP_TCP_TA_DATA_STALL_CYCLES_sum"],
+}
+
+
+def _pass_means(d):
+    """Per kernel label: mean of every counter of one --pmc run, plus the
+    mean dispatch duration in ns (from the rows' timestamps)."""
+    rows = [r for r in csv.DictReader(open(_csv(d, "counter_collection.csv"))) if "sph::" in r["Kernel_Name"]]
+    by_counter = collections.defaultdict(list)
+    for r in rows:
+        by_counter[r["Counter_Name"]].append(r)
+    out = collections.defaultdict(dict)
+    for cname, rs in by_counter.items():
+        acc = collections.defaultdict(list)
+        durs = collections.defaultdict(list)
+        for k, r in _full_grid(rs, lambda r: short_name(r["Kernel_Name"])):
+            acc[k].append(float(r["Counter_Value"]))
+            durs[k].append(_dur(r))
+        for k, v in acc.items():
+            out[k][cname] = sum(v) / len(v)
+            out[k]["_n"] = len(v)
+            out[k]["_ns"] = sum(durs[k]) / len(durs[k])
+    return out
+
+
+def units_summary(ta_td_dir, sq_dir, tcp_dir, cus=256, xcds=8):
+    """Per-kernel use of the vector-memory path (TA address unit, TD data
+    return, the L1 tag lookups) and of the VALU, from the three counter
+    passes of UNIT_PASSES.
+
+    GRBM_GUI_ACTIVE is read per XCD and summed over the `xcds` XCDs, so one
+    XCD's busy cycles are GRBM_GUI_ACTIVE / xcds (the derived metrics that
+    rocprofiler ships take its maximum over XCDs for the same reason); the
+    clock this implies (cycles / dispatch ns) is printed as a check. There
+    is one TA, one TD and one L1 (TCP) per CU. A SIMD issues one wave64 VALU
+    instruction per 4 cycles, so the CU's 4 SIMDs issue at most one per
+    cycle: VALU issue = SQ_INSTS_VALU / (cus x cycles)."""
+    passes = {"ta_td": _pass_means(ta_td_dir), "sq": _pass_means(sq_dir), "tcp": _pass_means(tcp_dir)}
+    kernels = set(passes["ta_td"]) & set(passes["sq"]) & set(passes["tcp"])
+
+    def cyc(p, k):
+        return passes[p][k]["GRBM_GUI_ACTIVE"] / xcds
+
+    out = ["| kernel | dispatches | us | clock GHz | TA busy | TA busy (max CU) | TA addr stalled by L1/L2 | "
+           "TD busy | TD stalled by L1/L2 | VALU issue | VALUBusy | VALU / wave | VMEM rd / wave | "
+           "VMEM wr / wave | L1 tag lookups / VMEM rd | L1 → L2 reads / dispatch | L1 pending-L2 stall |",
+           "|---|---|---|---|---|---|---|---|---|---|---|---|---|---|---|---|---|"]
+
+    def pct(x):
+        return f"{100.0 * x:.0f} %"
+
+    rows = []
+    for k in kernels:
+        a, s, t = passes["ta_td"][k], passes["sq"][k], passes["tcp"][k]
+        ca, cs, ct = cyc("ta_td", k), cyc("sq", k), cyc("tcp", k)
+        waves = max(s["SQ_WAVES"], 1.0)
+        rows.append((a["_ns"] * a["_n"], (
+            f"| {k} | {a['_n']} | {a['_ns'] / 1e3:.1f} | {ca / a['_ns']:.2f} | "
+            f"{pct(a['TA_BUSY_avr'] / ca)} | {pct(a['TA_BUSY_max'] / ca)} | "
+            f"{pct(a['TA_ADDR_STALLED_BY_TC_CYCLES_sum'] / cus / ca)} | "
+            f"{pct(a['TD_TD_BUSY_sum'] / cus / ca)} | {pct(a['TD_TC_STALL_sum'] / cus / ca)} | "
+            f"{pct(s['SQ_INSTS_VALU'] / (cus * cs))} | {pct(s['SQ_ACTIVE_INST_VALU'] / cus / cs)} | "
+            f"{s['SQ_INSTS_VALU'] / waves:.0f} | {s['SQ_INSTS_VMEM_RD'] / waves:.1f} | "
+            f"{s['SQ_INSTS_VMEM_WR'] / waves:.1f} | "
+            f"{t['TCP_TOTAL_CACHE_ACCESSES_sum'] / max(s['SQ_INSTS_VMEM_RD'], 1.0):.1f} | "
+            f"{t['TCP_TCC_READ_REQ_sum']:.0f} | {pct(t['TCP_PENDING_STALL_CYCLES_sum'] / cus / ct)} |")))
+    out += [r for _, r in sorted(rows, key=lambda x: -x[0])]
+    return "\n".join(out)
+
+
 def read_traffic(path, kernel):
     """Traffic bytes per dispatch of `kernel` from a pmc summary, or None."""
     if not os.path.exists(path):
@@ -276,9 +357,21 @@ def main(argv=None):
     tl.add_argument("--skip", type=int, default=10, help="steps dropped at the start (build, warm-up)")
     st = sub.add_parser("stats")
     st.add_argument("stats_csv", help="a *kernel_stats.csv or a directory holding one")
+    un = sub.add_parser("units")
+    un.add_argument("ta_td_dir", help="run with the counters of UNIT_PASSES['ta_td']")
+    un.add_argument("sq_dir", help="run with the counters of UNIT_PASSES['sq']")
+    un.add_argument("tcp_dir", help="run with the counters of UNIT_PASSES['tcp']")
+    un.add_argument("--cus", type=int, default=256)
+    un.add_argument("--xcds", type=int, default=8)
+    pc = sub.add_parser("passes", help="print the --pmc counter list of one units pass")
+    pc.add_argument("name", choices=sorted(UNIT_PASSES))
     a = ap.parse_args(argv)
     if a.cmd == "copy":
         copy_calibration(a.bytes)
+    elif a.cmd == "units":
+        print(units_summary(a.ta_td_dir, a.sq_dir, a.tcp_dir, a.cus, a.xcds))
+    elif a.cmd == "passes":
+        print(" ".join(UNIT_PASSES[a.name]))
     elif a.cmd == "trace":
         print(trace_summary(a.trace_dir, a.traffic))
     elif a.cmd == "timeline":
