@@ -150,16 +150,18 @@ def run_fast_math(n, steps, warmup, max_iter=None):
                     "tests/test_gpu_fast_math.py); the headline value is the IEEE build"}
 
 
-def run_batched(steps, warmup, plain_steps, count=512):
+def run_batched(steps, warmup, plain_steps, count=512, tune_cache=None):
     """Batched WCSPH: `count` seeded scenes (fluids_amd.scenes.c4_batched) in
     one packed batch, HIP-graph steps, autotuned launch shapes; against the
     same scenes as separate single-scene solvers stepped in turn (each on its
-    own stream, so their launches may overlap)."""
+    own stream, so their launches may overlap). With `tune_cache`, launch
+    shapes come from (or, when the file holds none for this batch, are tuned
+    and appended to) that file, so separate runs use the same shapes."""
     from fluids_amd import scenes
     scs = scenes.c4_batched(count=count)
     total = sum(sc.n_fluid for sc in scs)
     t0 = time.perf_counter()
-    b = scenes.build_batch(scs, autotune=True, trials=6)
+    b = scenes.build_batch(scs, autotune=True, cache=tune_cache, trials=6)
     t_tune = time.perf_counter() - t0
     b.step(warmup)
     b.synchronize()
@@ -259,6 +261,8 @@ def main():
     ap.add_argument("--batched-only", action="store_true", help="only the batched-scenes measurement")
     ap.add_argument("--batched-steps", type=int, default=50)
     ap.add_argument("--plain-steps", type=int, default=5)
+    ap.add_argument("--tune-cache", default=None,
+                    help="batched launch shapes: read from this file, or tuned and appended to it")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
     ap.add_argument("--no-fast-math", action="store_true", help="skip the SPH_FAST_MATH side run")
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint timing")
@@ -280,7 +284,7 @@ def main():
         print(json.dumps({"adjoint": run_adjoint()}), flush=True)
         return
     if a.batched_only:
-        print(json.dumps({"batched": run_batched(a.batched_steps, 5, a.plain_steps),
+        print(json.dumps({"batched": run_batched(a.batched_steps, 5, a.plain_steps, tune_cache=a.tune_cache),
                           "adjoint": None if a.no_adjoint else run_adjoint()}), flush=True)
         return
     t_init = time.perf_counter()
@@ -389,7 +393,7 @@ def main():
 
     batched = None
     if not a.no_batched:
-        batched = run_batched(a.batched_steps, 5, a.plain_steps)
+        batched = run_batched(a.batched_steps, 5, a.plain_steps, tune_cache=a.tune_cache)
     adjoint = None if a.no_adjoint else run_adjoint()
 
     rec = {

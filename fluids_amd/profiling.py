@@ -1,6 +1,6 @@
 """Summaries of rocprofv3 output for the solver's kernels.
 
-Two summaries are produced from rocprofv3's CSV output:
+Summaries produced from rocprofv3's CSV output:
 
 * ``trace``: per-kernel dispatch count and average / min / max duration from
   ``--kernel-trace`` (``*_kernel_trace.csv``), restricted to the dispatches at
@@ -249,6 +249,8 @@ def _pass_means(d):
             out[k][cname] = sum(v) / len(v)
             out[k]["_n"] = len(v)
             out[k]["_ns"] = sum(durs[k]) / len(durs[k])
+        for k, r in _full_grid(rs, lambda r: short_name(r["Kernel_Name"])):
+            out[k]["_shape"] = f"{_grid(r)}/{_wg(r)}"
     return out
 
 
@@ -270,10 +272,10 @@ def units_summary(ta_td_dir, sq_dir, tcp_dir, cus=256, xcds=8):
     def cyc(p, k):
         return passes[p][k]["GRBM_GUI_ACTIVE"] / xcds
 
-    out = ["| kernel | dispatches | us | clock GHz | TA busy | TA busy (max CU) | TA addr stalled by L1/L2 | "
+    out = ["| kernel | grid/block | dispatches | us | clock GHz | TA busy | TA busy (max CU) | TA addr stalled by L1/L2 | "
            "TD busy | TD stalled by L1/L2 | VALU issue | VALUBusy | VALU / wave | VMEM rd / wave | "
            "VMEM wr / wave | L1 tag lookups / VMEM rd | L1 → L2 reads / dispatch | L1 pending-L2 stall |",
-           "|---|---|---|---|---|---|---|---|---|---|---|---|---|---|---|---|---|"]
+           "|---|---|---|---|---|---|---|---|---|---|---|---|---|---|---|---|---|---|"]
 
     def pct(x):
         return f"{100.0 * x:.0f} %"
@@ -284,7 +286,7 @@ def units_summary(ta_td_dir, sq_dir, tcp_dir, cus=256, xcds=8):
         ca, cs, ct = cyc("ta_td", k), cyc("sq", k), cyc("tcp", k)
         waves = max(s["SQ_WAVES"], 1.0)
         rows.append((a["_ns"] * a["_n"], (
-            f"| {k} | {a['_n']} | {a['_ns'] / 1e3:.1f} | {ca / a['_ns']:.2f} | "
+            f"| {k} | {a['_shape']} | {a['_n']} | {a['_ns'] / 1e3:.1f} | {ca / a['_ns']:.2f} | "
             f"{pct(a['TA_BUSY_avr'] / ca)} | {pct(a['TA_BUSY_max'] / ca)} | "
             f"{pct(a['TA_ADDR_STALLED_BY_TC_CYCLES_sum'] / cus / ca)} | "
             f"{pct(a['TD_TD_BUSY_sum'] / cus / ca)} | {pct(a['TD_TC_STALL_sum'] / cus / ca)} | "
