@@ -1578,38 +1578,44 @@ void launch_fill_u32(void* p, size_t n, uint32_t v, hipStream_t s) {
 }
 
 // --------------------------------------------------------------- statistics
+// At most STATS_BLOCKS workgroups stride over the particles; each sums its
+// five counters in registers, across its waves in LDS, and adds them to acc
+// with one atomic per counter. (One atomic per wave put 5 x n/64 atomics on
+// acc's single cache line, and they execute one at a time: 0.94 ms at 1M.)
+constexpr int STATS_BLOCKS = 256;
+
 __global__ __launch_bounds__(256) void k_stats(int n, const int* __restrict__ nc, const int* __restrict__ bnc,
                                                const float4* __restrict__ pos, unsigned long long* __restrict__ acc) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long a = 0, b = 0, of = 0, ob = 0, nf = 0;
-    if (i < n) {
-        a = (unsigned long long)nc[i];
-        b = (unsigned long long)bnc[i];
-        of = nc[i] >= LIST_CAP;
-        ob = bnc[i] >= LIST_CAP;
-        float4 p = pos[i];
-        nf = !(isfinite(p.x) && isfinite(p.y) && isfinite(p.z));
+    __shared__ unsigned long long part[4][5];
+    unsigned long long v[5] = {0, 0, 0, 0, 0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int a = nc[i], b = bnc[i];
+        const float4 p = pos[i];
+        v[0] += (unsigned long long)a;
+        v[1] += (unsigned long long)b;
+        v[2] += a >= LIST_CAP;
+        v[3] += b >= LIST_CAP;
+        v[4] += !(isfinite(p.x) && isfinite(p.y) && isfinite(p.z));
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        a += __shfl_xor(a, off);
-        b += __shfl_xor(b, off);
-        of += __shfl_xor(of, off);
-        ob += __shfl_xor(ob, off);
-        nf += __shfl_xor(nf, off);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&acc[0], a);
-        atomicAdd(&acc[1], b);
-        atomicAdd(&acc[2], of);
-        atomicAdd(&acc[3], ob);
-        atomicAdd(&acc[4], nf);
+#pragma unroll
+    for (int c = 0; c < 5; ++c)
+        for (int off = 32; off > 0; off >>= 1) v[c] += __shfl_xor(v[c], off);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int c = 0; c < 5; ++c) part[threadIdx.x >> 6][c] = v[c];
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        unsigned long long t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += part[w][threadIdx.x];
+        atomicAdd(&acc[threadIdx.x], t);
     }
 }
 
 void launch_stats(int n, const int* nc, const int* bnc, const float4* pos, unsigned long long* acc, hipStream_t s) {
     launch_fill_u32(acc, 20, 0u, s);
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_stats, dim3((n + 255) / 256), dim3(256), 0, s, n, nc, bnc, pos, acc);
+    const int blocks = std::min((n + 255) / 256, STATS_BLOCKS);
+    hipLaunchKernelGGL(k_stats, dim3(blocks), dim3(256), 0, s, n, nc, bnc, pos, acc);
 }
 
 
