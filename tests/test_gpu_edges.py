@@ -46,6 +46,9 @@ def _step_and_compare(sc, method, steps, fields, label):
             assert nbad == 0, f"{label} step {step}: {f} has {nbad} differing words"
         st = sim.step_stats()
         assert st["fluid_list_overflows"] == int(np.sum(orc.field("ncount") >= 50)), label
+        assert st["boundary_list_overflows"] == int(np.sum(orc.field("bncount") >= 50)), label
+        assert st["fluid_neighbours"] == int(orc.field("ncount").astype(np.int64).sum()), label
+        assert st["boundary_neighbours"] == int(orc.field("bncount").astype(np.int64).sum()), label
         assert st["edge_reads"] == orc.edge_reads - before, label  # the oracle counts over its lifetime
     return sim, orc
 

@@ -68,6 +68,8 @@ def test_pcisph_1m_late_step_bitexact():
     st = sim.step_stats()
     assert st["iterations"] == orc.iterations
     assert st["boundary_neighbours"] > 300_000, st  # the fluid reaches the walls
+    assert st["fluid_neighbours"] == int(orc.field("ncount").astype(np.int64).sum())
+    assert st["boundary_neighbours"] == int(orc.field("bncount").astype(np.int64).sum())
     for f in INT_FIELDS:
         a, b = sim.field(f), orc.field(f)
         assert np.array_equal(a, b), f"late: {f} differs at {np.argwhere(a != b)[:5].ravel()}"
