@@ -122,3 +122,43 @@ def test_c4_full_batch_against_oracle_and_single_scenes():
         assert np.array_equal(b.half_velocities(k), s.half_velocities()), k
         s.close()
     b.close()
+
+
+def test_tiny_empty_and_clumped_scenes_in_a_batch():
+    """Scenes at the edges of the batched sort and walk, between ordinary
+    scenes: one particle, three, none, and 120 particles packed into a third
+    of a cell (lists at the cap; the clump flies apart, so most of its keys
+    change cell every step). Each non-empty scene must equal its single-scene
+    run bit for bit after every chunk of graph-replayed steps."""
+    base = _small_batch(4)
+    rng = np.random.default_rng(7)
+
+    def variant(sc, parts, name):
+        vols = [parts.astype(np.float32)] if len(parts) else []
+        return scenes.Scene(name, dict(sc.settings), sc.container, vols)
+
+    h = base[2].settings["fluid_support_radius"]
+    c = base[2].volumes[0][0, :3].astype(np.float64)
+    clump = np.concatenate([c + rng.uniform(0.0, h / 3, (120, 3)), np.ones((120, 1))], axis=1)
+    scs = [base[0], variant(base[0], base[0].volumes[0][:1], "one"),
+           variant(base[1], base[1].volumes[0][:3], "three"), variant(base[3], np.zeros((0, 4)), "empty"),
+           variant(base[2], clump, "clump"), base[3]]
+    b = scenes.build_batch(scs, autotune=False)
+    singles = {k: scenes.build(sc) for k, sc in enumerate(scs) if sc.n_fluid > 0}
+    assert b.total_particles == sum(sc.n_fluid for sc in scs)
+    for chunk, steps in enumerate([1, 10, 10, 9]):
+        b.step(steps)
+        for s in singles.values():
+            s.step(steps)
+        for k, s in singles.items():
+            label = f"chunk {chunk} scene {scs[k].name} (n={scs[k].n_fluid})"
+            assert np.array_equal(b.positions(k), s.positions()), label
+            assert np.array_equal(b.velocities(k), s.velocities()), label
+            assert np.array_equal(b.half_velocities(k), s.half_velocities()), label
+            assert np.array_equal(b.density(k), s.field("density")), label
+            assert np.array_equal(b.neighbour_counts(k), s.field("ncount")), label
+        if chunk == 0:
+            assert int(b.neighbour_counts(4).max()) == 50  # the clump's lists reach the cap
+    for s in singles.values():
+        s.close()
+    b.close()
