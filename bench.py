@@ -242,6 +242,91 @@ def run_adjoint(repeats=3):
             "backward_over_plain_forward": tb / tp, "forward_plus_backward_over_plain_forward": (tf + tb) / tp}
 
 
+def run_next_rows(steps=50, frame_steps=20):
+    """Side figures of the code around the path (not the value): the rigid
+    coupling loop on the configured double dam break (0.01 m lattice, the
+    12 kg cube; sph_coupled_step reads the body wrench every step, so the
+    plain step is also synchronised every step for the comparison); the 1M
+    throughput scene with its positions written to a frame file after every
+    step (after the writer's first frames, which allocate its pinned
+    slots), against the same steps without; the reference's many_objects.json
+    scene file loaded and built (samplers and the PCISPH relaxation step
+    included), and the teapot mesh sampled at 0.01 m."""
+    import os
+    import tempfile
+    from fluids_amd import load_scene, load_settings, model_sampling, scenes
+    from fluids_amd.output import FrameWriter
+    from fluids_amd.rigid import RigidWorld, coupled_step
+    golden = os.path.join(ROOT, "tests", "golden")
+    out = {}
+
+    sc = scenes.c2_pcisph_double_dambreak(spacing=0.01)
+    sim = scenes.build(sc)
+    world = RigidWorld(-9.8)
+    world.set_container(*sc.container)
+    cube = sc.bodies[0]
+    pairs = [(0, world.add_cube(0.3, cube.mass, cube.com, (0, 1, 0, 1)))]
+    for _ in range(5):
+        coupled_step(sim, world, pairs)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        coupled_step(sim, world, pairs)
+    sim.synchronize()
+    t_coupled = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step(1)
+        sim.synchronize()
+    t_plain = (time.perf_counter() - t0) / steps
+    out["coupled_double_dambreak"] = {"fluid_particles": sc.n_fluid, "boundary_particles": sc.n_boundary,
+                                      "steps": steps, "coupled_ms_per_step": 1e3 * t_coupled,
+                                      "plain_synchronised_ms_per_step": 1e3 * t_plain,
+                                      "coupling_overhead": t_coupled / t_plain - 1.0}
+    sim.close()
+    world.close()
+
+    sc = scenes.c3_pcisph_throughput(1_000_000)
+    sim = scenes.build(sc)
+    sim.step(5)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.step(frame_steps)
+    sim.synchronize()
+    t_steps = (time.perf_counter() - t0) / frame_steps
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "frames.bin")
+        fw = FrameWriter(sim, path, positions=True)
+        for _ in range(3):  # the writer's three pinned slots are allocated by its first frames
+            sim.step(1)
+            fw.write()
+        sim.synchronize()
+        time.sleep(0.2)
+        t0 = time.perf_counter()
+        for _ in range(frame_steps):
+            sim.step(1)
+            fw.write()
+        frames = fw.close() - 3
+        t_frames = (time.perf_counter() - t0) / frame_steps
+        size = os.path.getsize(path)
+    out["frames_1000k"] = {"steps": frame_steps, "frames": frames, "bytes_per_frame": (size - 16) / (frames + 3),
+                           "step_ms": 1e3 * t_steps, "step_and_frame_ms": 1e3 * t_frames}
+    sim.close()
+
+    t0 = time.perf_counter()
+    phys, simset = load_settings(os.path.join(golden, "config.pci"))
+    ls = load_scene(os.path.join(golden, "many_objects.json"), phys, simset)
+    ls.simulation.synchronize()
+    t_load = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    teapot = model_sampling(os.path.join(golden, "teapot.obj"), 0.01)
+    t_model = time.perf_counter() - t0
+    out["scene_front_end"] = {"many_objects_load_and_build_s": t_load,
+                              "fluid_particles": ls.simulation.particle_count(),
+                              "teapot_samples": int(teapot.shape[0]), "teapot_sampling_s": t_model}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -266,6 +351,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
     ap.add_argument("--no-fast-math", action="store_true", help="skip the SPH_FAST_MATH side run")
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint timing")
+    ap.add_argument("--next-rows", action="store_true",
+                    help="only the side figures of the rigid coupling, frame output and scene loading")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "pmc_pcisph_throughput_1000k.md"),
                     help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary (python -m fluids_amd.profiling pmc) "
                          "that gives the dominant kernel's HBM bytes per launch")
@@ -282,6 +369,9 @@ def main():
     lib()
     if a.adjoint_only:
         print(json.dumps({"adjoint": run_adjoint()}), flush=True)
+        return
+    if a.next_rows:
+        print(json.dumps({"next_rows": run_next_rows()}), flush=True)
         return
     if a.batched_only:
         print(json.dumps({"batched": run_batched(a.batched_steps, 5, a.plain_steps, tune_cache=a.tune_cache),
