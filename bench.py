@@ -324,6 +324,7 @@ def run_next_rows(steps=50, frame_steps=20):
     out["scene_front_end"] = {"many_objects_load_and_build_s": t_load,
                               "fluid_particles": ls.simulation.particle_count(),
                               "teapot_samples": int(teapot.shape[0]), "teapot_sampling_s": t_model}
+    ls.close()
     return out
 
 
