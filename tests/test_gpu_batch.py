@@ -162,3 +162,32 @@ def test_tiny_empty_and_clumped_scenes_in_a_batch():
     for s in singles.values():
         s.close()
     b.close()
+
+
+def test_batch_moved_keys_sort_matches_full_sort_singles_over_200_steps():
+    """The batch sorts only the keys that changed cell after its first step;
+    the same scenes run one at a time with SPH_FULL_SORT (the radix sort of
+    every key every step) must give the same bits after every 50 of 200
+    graph-replayed steps."""
+    import dataclasses
+    scs = _small_batch(24)
+    b = scenes.build_batch(scs, autotune=False)
+    singles = []
+    for sc in scs:
+        st = dict(sc.settings)
+        st["flags"] = st.get("flags", 0) | 4  # SPH_FULL_SORT
+        singles.append(scenes.build(dataclasses.replace(sc, settings=st)))
+    for chunk in range(4):
+        b.step(50)
+        for s in singles:
+            s.step(50)
+        for k, s in enumerate(singles):
+            label = f"step {50 * (chunk + 1)} scene {k} (n={scs[k].n_fluid})"
+            assert np.array_equal(b.positions(k), s.positions()), label
+            assert np.array_equal(b.velocities(k), s.velocities()), label
+            assert np.array_equal(b.half_velocities(k), s.half_velocities()), label
+    for k, s in enumerate(singles):
+        assert np.array_equal(b.density(k), s.field("density")), k
+        assert np.array_equal(b.neighbour_counts(k), s.field("ncount")), k
+        s.close()
+    b.close()
