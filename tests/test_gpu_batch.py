@@ -191,3 +191,33 @@ def test_batch_moved_keys_sort_matches_full_sort_singles_over_200_steps():
         assert np.array_equal(b.neighbour_counts(k), s.field("ncount")), k
         s.close()
     b.close()
+
+
+def test_mixed_density_forms_in_one_batch():
+    """Half the scenes keep SPH_FIX_WCSPH_DENSITY (W_poly6 evaluated on r^2),
+    half use the reference's own form (W_poly6 evaluated on r,
+    wcsph.cl:61-62,70-71); the batched walk picks the form per scene, and
+    every scene must equal its single-scene run."""
+    import dataclasses
+    base = _small_batch(8)
+    scs = []
+    for k, sc in enumerate(base):
+        st = dict(sc.settings)
+        if k % 2:
+            st["flags"] = st.get("flags", 0) & ~1  # SPH_FIX_WCSPH_DENSITY off
+        scs.append(dataclasses.replace(sc, settings=st))
+    assert {sc.settings["flags"] & 1 for sc in scs} == {0, 1}
+    b = scenes.build_batch(scs, autotune=False)
+    singles = [scenes.build(sc) for sc in scs]
+    for chunk in range(3):
+        b.step(10)
+        for s in singles:
+            s.step(10)
+        for k, s in enumerate(singles):
+            label = f"chunk {chunk} scene {k} flags {scs[k].settings['flags']}"
+            assert np.array_equal(b.positions(k), s.positions()), label
+            assert np.array_equal(b.velocities(k), s.velocities()), label
+            assert np.array_equal(b.density(k), s.field("density")), label
+    for s in singles:
+        s.close()
+    b.close()
